@@ -1,0 +1,137 @@
+/*
+ * pbf.h -- C ABI of the MI355X PBF substep library (libpbf_amd.so).
+ *
+ * The library replaces the per-substep kernel chain that the reference's
+ * Simulation class drives (src/Simulation.hpp:105-150,
+ * src/Simulation.cpp:554-621).  Each entry point names the reference
+ * interface it stands in for.  Conventions:
+ *   - every function returns PBF_OK (0) or a negative PBF_E* code; no C++
+ *     exception crosses this boundary; pbf_last_error() gives the message;
+ *   - the library owns device memory, the caller owns host pointers;
+ *   - one HIP stream per context; a context is not thread-safe;
+ *   - pbf_step is asynchronous; the getters synchronise the stream.
+ * Arrays of float4 are float[4*n], particle-major (x, y, z, w).
+ */
+#ifndef PBF_PBF_H
+#define PBF_PBF_H
+
+#include <stdint.h>
+#include "params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBF_OK 0
+#define PBF_E_INVALID (-1)   /* bad argument or parameter                  */
+#define PBF_E_HIP (-2)       /* HIP runtime error (message has the detail) */
+#define PBF_E_NOMEM (-3)     /* device allocation failed                   */
+#define PBF_E_STATE (-4)     /* call out of order (e.g. step before state) */
+
+/* Error/diagnostic flags accumulated on the device (pbf_error_flags). */
+#define PBF_FLAG_SORT_TIMEOUT 1u   /* radix-sort look-back wait hit its bound    */
+#define PBF_FLAG_FRIENDS_OVERFLOW 2u /* a circle held 50 friends; extras dropped  */
+#define PBF_FLAG_NAN 4u            /* non-finite predicted position seen        */
+
+typedef struct pbf_ctx pbf_ctx;
+
+/* Settings with no field in the reference Parameters block. */
+typedef struct pbf_options {
+    float gravity[3];          /* used when use_gravity_vector != 0          */
+    uint32_t use_gravity_vector; /* 0: gravity = (0, -garvity, 0) as in
+                                  predict_positions.cl:13                    */
+    uint32_t walls;            /* 1: six box planes at x/y/zMin/Max in the
+                                  delta stage; 0: no walls                   */
+    uint32_t reserved[8];
+} pbf_options;
+
+/* Fills *o with the defaults: reference gravity, walls on. */
+void pbf_default_options(pbf_options *o);
+
+/* Sets h_2 = h*h, friendsCircles = 5, particlesPerCircle = 50
+ * (src/ParamUtils.cpp:97-99). */
+void pbf_params_finalize(pbf_params *p);
+
+/* Parses the reference's .par text (src/ParamUtils.cpp:24-100: '#'
+ * comments, case-insensitive keys, unknown keys skipped) into *p and
+ * finalizes it.  Returns the number of unknown keys, or a PBF_E* code. */
+int pbf_load_parameters(const char *text, pbf_params *p);
+
+/* Simulation::Simulation + InitBuffers + InitCells + InitKernels
+ * (src/Simulation.cpp:56-63,206-255,129-204): allocates every device
+ * buffer for params->particleCount particles. opt may be NULL. */
+int pbf_create(const pbf_params *params, const pbf_options *opt, pbf_ctx **out);
+
+/* Initial particle state, the role of Simulation::CreateParticles
+ * (src/Simulation.cpp:71-101): n must equal particleCount; v may be NULL
+ * (zero velocities). */
+int pbf_set_state(pbf_ctx *ctx, const float *x4, const float *v4, uint32_t n);
+
+/* Simulation::Step (src/Simulation.cpp:554-621).  pause mirrors bPauseSim:
+ * no gravity and no re-sort. */
+int pbf_step(pbf_ctx *ctx, uint32_t pause);
+
+/* steps x Simulation::Step, as Runner::run does per frame
+ * (src/Runner.cpp:124-132). */
+int pbf_run(pbf_ctx *ctx, uint32_t steps, uint32_t pause);
+
+/* Blocks until all queued work of the context is done. */
+int pbf_sync(pbf_ctx *ctx);
+
+/* State after the last step, in the current (cell-sorted) particle order.
+ * x4: positions, w = |v| (the render VBO, update_velocities.cl:14);
+ * v4: velocities; xs4: predicted positions, w = density (the render
+ * texture, update_velocities.cl:16-17); rho, lambda: last solver
+ * iteration; omega4: vorticity.  Any pointer may be NULL. */
+int pbf_get_state(pbf_ctx *ctx, float *x4, float *v4, float *xs4, float *rho, float *lambda, float *omega4);
+
+/* Structure of the last step: sorted cell keys, the permutation applied by
+ * the sort (radixsort.cl:22-37), cells (uint32[2*gridBufSize],
+ * update_cells.cl) and the friends list in the reference layout
+ * (build_friends_list.cl:1-21: counts[5][n] then indices[5][50][n]).
+ * Any pointer may be NULL. */
+int pbf_get_structure(pbf_ctx *ctx, uint32_t *keys_sorted, uint32_t *perm, uint32_t *cells, uint32_t *friends);
+
+/* Device pointers for zero-copy interop (the shared GL buffers of
+ * src/Simulation.cpp:209-211 on the reference side). */
+typedef struct pbf_dev_bufs {
+    void *positions;   /* float4[n], w = |v|       */
+    void *predicted;   /* float4[n], w = density   */
+    void *velocities;  /* float4[n]                */
+    void *density;     /* float[n]                 */
+    void *lambda;      /* float[n]                 */
+    void *omega;       /* float4[n]                */
+    void *stream;      /* hipStream_t of the context */
+    uint32_t n;
+} pbf_dev_bufs;
+int pbf_device_buffers(pbf_ctx *ctx, pbf_dev_bufs *out);
+
+/* Per-stage timing, the OCLPerfMon trackers (src/OCLPerfMon.cpp:3-50):
+ * same names, "_<iteration>" suffix, EMA weight 0.5.  Enabled with
+ * pbf_set_profiling(ctx, 1); pbf_timings synchronises. */
+typedef struct pbf_timing {
+    char name[48];
+    double last_ms;    /* last measured duration                    */
+    double ema_ms;     /* total_time of OCLPerfMon (weight 0.5)      */
+    double sum_ms;     /* accumulated since profiling was enabled    */
+    uint64_t calls;
+} pbf_timing;
+int pbf_set_profiling(pbf_ctx *ctx, int enable);
+int pbf_timings(pbf_ctx *ctx, pbf_timing *out, int cap, int *count);
+
+/* Accumulated PBF_FLAG_* bits and the number of dropped friends; reset
+ * clears them.  Synchronises. */
+int pbf_error_flags(pbf_ctx *ctx, uint32_t *flags, uint64_t *dropped_friends, int reset);
+
+void pbf_destroy(pbf_ctx *ctx);
+
+/* Message of the last failing call on ctx (or of the last pbf_create
+ * failure when ctx is NULL). */
+const char *pbf_last_error(const pbf_ctx *ctx);
+
+const char *pbf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,165 @@
+"""ctypes binding of libpbf_amd.so (the C ABI in include/pbf/pbf.h).
+
+The library is loaded from position-based-fluids_amd/lib/; if it is missing
+this module raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_ROOT / "lib" / "libpbf_amd.so"
+
+PBF_OK = 0
+PBF_E_INVALID = -1
+PBF_E_HIP = -2
+PBF_E_NOMEM = -3
+PBF_E_STATE = -4
+
+PBF_FLAG_SORT_TIMEOUT = 1
+PBF_FLAG_FRIENDS_OVERFLOW = 2
+PBF_FLAG_NAN = 4
+
+
+class PbfParams(ctypes.Structure):
+    """Mirror of `struct Parameters` (src/Parameters.hpp:5-56), 124 bytes."""
+
+    _fields_ = [
+        ("resetSimOnChange", ctypes.c_int32),
+        ("particleCount", ctypes.c_uint32),
+        ("xMin", ctypes.c_float),
+        ("xMax", ctypes.c_float),
+        ("yMin", ctypes.c_float),
+        ("yMax", ctypes.c_float),
+        ("zMin", ctypes.c_float),
+        ("zMax", ctypes.c_float),
+        ("waveGenAmp", ctypes.c_float),
+        ("waveGenFreq", ctypes.c_float),
+        ("waveGenDuty", ctypes.c_float),
+        ("timeStep", ctypes.c_float),
+        ("simIterations", ctypes.c_uint32),
+        ("subSteps", ctypes.c_uint32),
+        ("h", ctypes.c_float),
+        ("restDensity", ctypes.c_float),
+        ("epsilon", ctypes.c_float),
+        ("garvity", ctypes.c_float),
+        ("vorticityFactor", ctypes.c_float),
+        ("viscosityFactor", ctypes.c_float),
+        ("surfaceTenstionK", ctypes.c_float),
+        ("surfaceTenstionDist", ctypes.c_float),
+        ("friendsCircles", ctypes.c_uint32),
+        ("particlesPerCircle", ctypes.c_uint32),
+        ("gridBufSize", ctypes.c_uint32),
+        ("setupSpacing", ctypes.c_float),
+        ("segmentSize", ctypes.c_uint32),
+        ("sortIterations", ctypes.c_uint32),
+        ("particleRenderSize", ctypes.c_float),
+        ("h_2", ctypes.c_float),
+        ("EnableCachedBuffers", ctypes.c_bool),
+    ]
+
+    def copy(self) -> "PbfParams":
+        out = PbfParams()
+        ctypes.memmove(ctypes.byref(out), ctypes.byref(self), ctypes.sizeof(PbfParams))
+        return out
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class PbfOptions(ctypes.Structure):
+    _fields_ = [
+        ("gravity", ctypes.c_float * 3),
+        ("use_gravity_vector", ctypes.c_uint32),
+        ("walls", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 8),
+    ]
+
+
+class PbfTiming(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 48),
+        ("last_ms", ctypes.c_double),
+        ("ema_ms", ctypes.c_double),
+        ("sum_ms", ctypes.c_double),
+        ("calls", ctypes.c_uint64),
+    ]
+
+
+class PbfDevBufs(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.c_void_p),
+        ("predicted", ctypes.c_void_p),
+        ("velocities", ctypes.c_void_p),
+        ("density", ctypes.c_void_p),
+        ("lambda_", ctypes.c_void_p),
+        ("omega", ctypes.c_void_p),
+        ("stream", ctypes.c_void_p),
+        ("n", ctypes.c_uint32),
+    ]
+
+
+class PbfError(RuntimeError):
+    """A PBF_E_* return code; the message is pbf_last_error()."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"pbf error {code}: {message}")
+        self.code = code
+
+
+class LibraryMissing(ImportError):
+    pass
+
+
+_LIB = None
+
+_VP = ctypes.c_void_p
+_FP = ctypes.POINTER(ctypes.c_float)
+_UP = ctypes.POINTER(ctypes.c_uint32)
+
+_SIGNATURES = {
+    "pbf_version": (ctypes.c_char_p, []),
+    "pbf_default_options": (None, [ctypes.POINTER(PbfOptions)]),
+    "pbf_params_finalize": (None, [ctypes.POINTER(PbfParams)]),
+    "pbf_load_parameters": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(PbfParams)]),
+    "pbf_create": (ctypes.c_int, [ctypes.POINTER(PbfParams), ctypes.POINTER(PbfOptions), ctypes.POINTER(_VP)]),
+    "pbf_set_state": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32]),
+    "pbf_step": (ctypes.c_int, [_VP, ctypes.c_uint32]),
+    "pbf_run": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32]),
+    "pbf_sync": (ctypes.c_int, [_VP]),
+    "pbf_get_state": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "pbf_get_structure": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "pbf_device_buffers": (ctypes.c_int, [_VP, ctypes.POINTER(PbfDevBufs)]),
+    "pbf_set_profiling": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "pbf_timings": (ctypes.c_int, [_VP, ctypes.POINTER(PbfTiming), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "pbf_error_flags": (ctypes.c_int, [_VP, _UP, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),
+    "pbf_destroy": (None, [_VP]),
+    "pbf_last_error": (ctypes.c_char_p, [_VP]),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libpbf_amd.so once; raises LibraryMissing if it was not built."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise LibraryMissing(
+                f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')"
+            )
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != PBF_OK:
+        msg = lib().pbf_last_error(ctx)
+        raise PbfError(rc, msg.decode() if msg else "")
+
+
+assert ctypes.sizeof(PbfParams) == 124, ctypes.sizeof(PbfParams)

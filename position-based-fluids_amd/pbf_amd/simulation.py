@@ -1,0 +1,154 @@
+"""Host-side mirror of the reference's Simulation class over the C ABI.
+
+Method names follow src/Simulation.hpp:105-150 (InitBuffers, InitCells,
+InitKernels, Step) and the public state flags (bPauseSim); the context is
+created lazily by the first of InitBuffers/InitCells/InitKernels, which the
+library performs as one call (pbf_create).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import PbfDevBufs, PbfError, PbfOptions, PbfParams, PbfTiming, check, lib
+from .scenes import default_options
+
+_CIRCLES = 5
+_PER_CIRCLE = 50
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Simulation:
+    def __init__(self, params: PbfParams, options: PbfOptions | None = None):
+        self.params = params.copy()
+        self.options = options if options is not None else default_options()
+        self._ctx = None
+        self.bPauseSim = False
+        self.bReadFriendsList = False
+        self.fWavePos = 0.0  # passed to computeDelta but unused there (compute_delta.cl:50)
+
+    # -- Simulation::InitBuffers / InitCells / InitKernels ------------------
+    def _ensure(self):
+        if self._ctx is None:
+            ctx = ctypes.c_void_p()
+            rc = lib().pbf_create(self.params, self.options, ctypes.byref(ctx))
+            if rc != 0:
+                raise PbfError(rc, lib().pbf_last_error(None).decode())
+            self._ctx = ctx
+
+    def InitBuffers(self):
+        self._ensure()
+
+    def InitCells(self):
+        self._ensure()
+
+    def InitKernels(self) -> bool:
+        self._ensure()
+        return True
+
+    # -- state ----------------------------------------------------------------
+    @property
+    def n(self) -> int:
+        return int(self.params.particleCount)
+
+    def set_state(self, x: np.ndarray, v: np.ndarray | None = None):
+        self._ensure()
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 4)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 4)
+        check(lib().pbf_set_state(self._ctx, _ptr(x), _ptr(vv), x.shape[0]), self._ctx)
+
+    def Step(self):
+        """Simulation::Step (src/Simulation.cpp:554-621), asynchronous."""
+        check(lib().pbf_step(self._ctx, 1 if self.bPauseSim else 0), self._ctx)
+
+    def run(self, steps: int):
+        check(lib().pbf_run(self._ctx, steps, 1 if self.bPauseSim else 0), self._ctx)
+
+    def sync(self):
+        check(lib().pbf_sync(self._ctx), self._ctx)
+
+    def get_state(self) -> dict:
+        n = self.n
+        out = {
+            "x": np.empty((n, 4), np.float32),
+            "v": np.empty((n, 4), np.float32),
+            "xs": np.empty((n, 4), np.float32),
+            "rho": np.empty(n, np.float32),
+            "lambda": np.empty(n, np.float32),
+            "omega": np.empty((n, 4), np.float32),
+        }
+        check(lib().pbf_get_state(self._ctx, _ptr(out["x"]), _ptr(out["v"]), _ptr(out["xs"]), _ptr(out["rho"]),
+                                  _ptr(out["lambda"]), _ptr(out["omega"])), self._ctx)
+        return out
+
+    def get_structure(self, friends: bool = True) -> dict:
+        n = self.n
+        out = {
+            "keys": np.empty(n, np.uint32),
+            "perm": np.empty(n, np.uint32),
+            "cells": np.empty(2 * int(self.params.gridBufSize), np.uint32),
+        }
+        fr = np.empty(n * _CIRCLES * (1 + _PER_CIRCLE), np.uint32) if friends else None
+        check(lib().pbf_get_structure(self._ctx, _ptr(out["keys"]), _ptr(out["perm"]), _ptr(out["cells"]), _ptr(fr)),
+              self._ctx)
+        if friends:
+            out["friends"] = fr
+        return out
+
+    def device_buffers(self) -> PbfDevBufs:
+        b = PbfDevBufs()
+        check(lib().pbf_device_buffers(self._ctx, ctypes.byref(b)), self._ctx)
+        return b
+
+    # -- OCLPerfMon-style telemetry -------------------------------------------
+    def set_profiling(self, enable: bool):
+        check(lib().pbf_set_profiling(self._ctx, 1 if enable else 0), self._ctx)
+
+    def timings(self) -> dict:
+        cnt = ctypes.c_int()
+        check(lib().pbf_timings(self._ctx, None, 0, ctypes.byref(cnt)), self._ctx)
+        arr = (PbfTiming * max(cnt.value, 1))()
+        check(lib().pbf_timings(self._ctx, arr, cnt.value, ctypes.byref(cnt)), self._ctx)
+        return {
+            t.name.decode(): {"last_ms": t.last_ms, "ema_ms": t.ema_ms, "sum_ms": t.sum_ms, "calls": int(t.calls)}
+            for t in arr[: cnt.value]
+        }
+
+    def error_flags(self, reset: bool = False) -> tuple[int, int]:
+        f = ctypes.c_uint32()
+        d = ctypes.c_uint64()
+        check(lib().pbf_error_flags(self._ctx, ctypes.byref(f), ctypes.byref(d), 1 if reset else 0), self._ctx)
+        return int(f.value), int(d.value)
+
+    def close(self):
+        if self._ctx is not None:
+            lib().pbf_destroy(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        self._ensure()
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_parameters(text: str) -> tuple[PbfParams, int]:
+    """LoadParameters (src/ParamUtils.cpp:24-100) through the C ABI.
+
+    Returns the parameters and the number of unknown keys."""
+    p = PbfParams()
+    rc = lib().pbf_load_parameters(text.encode(), p)
+    if rc < 0:
+        raise PbfError(rc, "pbf_load_parameters failed")
+    return p, rc

@@ -1,0 +1,219 @@
+"""GPU parity: the HIP substep (through the C ABI) against the CPU oracle.
+
+Integer structure (cell keys, sort permutation, cells, friends lists) must
+match bit for bit.  Floating-point state must agree within 1e-5*h in
+positions (1e-5*h/dt in velocities); both sides evaluate the same binary32
+operations in the same order without contraction, so the state is also
+checked for bitwise equality.
+"""
+import numpy as np
+import pytest
+
+import pbf_amd
+import pbf_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def make_pair(scene):
+    sim = pbf_amd.Simulation(scene.params, scene.options)
+    sim.set_state(scene.x, scene.v)
+    ref = pbf_oracle.OracleSim(pbf_oracle.cfg_from_params(scene.params, scene.options))
+    ref.set_state(scene.x, scene.v)
+    return sim, ref
+
+
+def advance(sim, ref, steps=1, pauses=None):
+    for s in range(steps):
+        pause = bool(pauses[s]) if pauses is not None else False
+        sim.bPauseSim = pause
+        sim.Step()
+        ref.step(pause)
+
+
+def check_structure(sim, ref, friends=True):
+    a = sim.get_structure(friends=friends)
+    b = ref.get_structure(friends=friends)
+    assert np.array_equal(a["keys"], b["keys"]), "cell keys differ"
+    assert np.array_equal(a["perm"], b["perm"]), "sort permutation differs"
+    assert np.array_equal(a["cells"], b["cells"]), "cell start/end differ"
+    if friends:
+        assert np.array_equal(a["friends"], b["friends"]), "friends lists differ"
+    return a
+
+
+def check_state(sim, ref, scene, bitwise=True):
+    a = sim.get_state()
+    b = ref.get_state()
+    h = float(scene.params.h)
+    dt = float(scene.params.timeStep)
+    for k, tol in (("x", 1e-5 * h), ("xs", 1e-5 * h), ("v", 1e-5 * h / dt), ("omega", None)):
+        da, db = a[k][:, :3], b[k][:, :3]
+        assert np.array_equal(np.isnan(da), np.isnan(db)), f"{k}: NaN pattern differs"
+        fin = ~np.isnan(db)
+        err = float(np.max(np.abs(da[fin] - db[fin]))) if fin.any() else 0.0
+        if tol is not None:
+            assert err <= tol, f"{k}: max |diff| {err:.3e} > {tol:.3e}"
+        if bitwise:
+            assert np.array_equal(da.view(np.uint32)[fin], db.view(np.uint32)[fin]), f"{k}: not bitwise equal (max {err:.3e})"
+    for k in ("rho", "lambda"):
+        if bitwise:
+            fin = ~np.isnan(b[k])
+            assert np.array_equal(a[k].view(np.uint32)[fin], b[k].view(np.uint32)[fin]), f"{k} not bitwise equal"
+    return a, b
+
+
+@pytest.mark.parametrize("name", ["dam4k", "dam16k"])
+def test_one_step(name):
+    scene = pbf_amd.config(name, seed=3)
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 1)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    flags, dropped = sim.error_flags()
+    assert flags == 0 and dropped == 0
+    sim.close()
+
+
+def test_ten_steps_dam16k():
+    scene = pbf_amd.config("dam16k", seed=5)
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 10)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()
+
+
+def test_ragged_count():
+    # 3,000 particles: not a multiple of the 256-thread blocks or 2,048-key sort tiles
+    scene = pbf_amd.dam_break(15, 20, 10, 0.1, (1.6, 2.4, 1.6), seed=7)
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 3)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()
+
+
+@pytest.mark.parametrize("gbs", [1, 64, 97, 4093])
+def test_hash_folding(gbs):
+    # a small GRID_BUF_SIZE folds distinct cells onto one hash: neighbour
+    # cells are visited once per alias, so friends lists carry duplicates
+    scene = pbf_amd.dam_break(12, 12, 12, 0.1, (1.2, 1.2, 1.2), seed=11, grid_buf_size=gbs)
+    if gbs == 1:
+        # every particle shares one cell: 27 visits of all particles, keep it tiny
+        scene = pbf_amd.dam_break(4, 4, 4, 0.1, (0.4, 0.4, 0.4), seed=11, grid_buf_size=gbs)
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 2)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()
+
+
+@pytest.mark.parametrize("n", [1, 2, 65])
+def test_tiny_counts(n):
+    x = pbf_amd.dam_break_block(n, 1, 1, 0.1, seed=2)
+    x[:, 1] += 0.3
+    p = pbf_amd.physical_params(n, 0.1, (0, 0, 0), (8.0, 1.0, 1.0))
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(), x, np.zeros_like(x))
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 3)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()
+
+
+def test_pause_semantics():
+    scene = pbf_amd.config("dam4k", seed=9)
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 5, pauses=[1, 0, 1, 1, 0])
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()
+
+
+def test_friend_overflow_flag():
+    # 400 particles within 0.2h of each other: every pair lands in circle 0,
+    # which keeps the first 50 in scan order and drops the rest
+    rng = np.random.default_rng(0)
+    n = 400
+    x = np.zeros((n, 4), np.float32)
+    x[:, :3] = (0.55 + 0.02 * rng.random((n, 3))).astype(np.float32)
+    p = pbf_amd.physical_params(n, 0.1, (0, 0, 0), (1.0, 1.0, 1.0), iters=1)
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(), x, np.zeros_like(x))
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 1)
+    st = check_structure(sim, ref)
+    flags, dropped = sim.error_flags(reset=True)
+    counts = st["friends"][: 5 * n].reshape(5, n)
+    assert flags & pbf_amd.PBF_FLAG_FRIENDS_OVERFLOW
+    assert int(counts[0].max()) == 50
+    # every particle sees n-1 friends in range; all beyond 50 per circle drop
+    assert dropped == int(n * (n - 1) - counts.sum())
+    assert sim.error_flags() == (0, 0)
+    sim.close()
+
+
+def test_deterministic_repeat():
+    scene = pbf_amd.config("dam16k", seed=4)
+    outs = []
+    for _ in range(2):
+        sim = pbf_amd.Simulation(scene.params, scene.options)
+        sim.set_state(scene.x, scene.v)
+        sim.run(8)
+        outs.append(sim.get_state())
+        sim.close()
+    for k in ("x", "v", "xs", "omega"):
+        assert np.array_equal(outs[0][k].view(np.uint32), outs[1][k].view(np.uint32)), k
+
+
+def _traces(state, rest_density):
+    rho = state["rho"].astype(np.float64)
+    v = state["v"][:, :3].astype(np.float64)
+    return float(np.mean(np.abs(rho / rest_density - 1.0))), float(0.5 * np.sum(v * v))
+
+
+def test_trace_200_steps_dam16k():
+    """Density-error and kinetic-energy traces over 200 substeps within 1%."""
+    scene = pbf_amd.config("dam16k", seed=0)
+    sim, ref = make_pair(scene)
+    rd = float(scene.params.restDensity)
+    worst = 0.0
+    for s in range(200):
+        advance(sim, ref, 1)
+        if s % 10 == 9:
+            da, ka = _traces(sim.get_state(), rd)
+            db, kb = _traces(ref.get_state(), rd)
+            worst = max(worst, abs(da - db) / max(abs(db), 1e-30), abs(ka - kb) / max(abs(kb), 1e-30))
+    assert worst <= 0.01, worst
+    flags, _ = sim.error_flags()
+    assert not (flags & pbf_amd.PBF_FLAG_SORT_TIMEOUT)
+    sim.close()
+
+
+def test_one_step_dam1m():
+    """Full-size workload: structure bit-exact, state within tolerance."""
+    scene = pbf_amd.config("dam1m", seed=0)
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 1)
+    a = check_structure(sim, ref)
+    n = scene.n
+    keys = a["keys"]
+    assert np.all(keys[1:] >= keys[:-1])
+    assert np.array_equal(np.sort(a["perm"]), np.arange(n, dtype=np.uint32))
+    check_state(sim, ref, scene)
+    flags, dropped = sim.error_flags()
+    assert flags == 0 and dropped == 0
+    sim.close()
+
+
+def test_profiling_trackers():
+    scene = pbf_amd.config("dam4k", seed=1)
+    sim = pbf_amd.Simulation(scene.params, scene.options)
+    sim.set_state(scene.x, scene.v)
+    sim.set_profiling(True)
+    sim.run(3)
+    t = sim.timings()
+    for name in ("predictPositions", "onesweep_0", "sortParticles", "updateCells", "buildFriendsList",
+                 "computeScaling_0", "computeDelta_3", "applyViscosity", "applyVorticity"):
+        assert name in t and t[name]["calls"] == 3 and t[name]["last_ms"] > 0.0, name
+    sim.close()

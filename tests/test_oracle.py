@@ -1,0 +1,353 @@
+"""Pins the CPU oracle against closed forms, brute force and invariants.
+
+The reference ships no tests, fixtures or golden vectors; these checks fix
+the restatement where the mathematics or an independent computation does.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import pbf_amd
+import pbf_oracle as po
+
+
+# --- Morton hash (utilities.cl:69-87) ----------------------------------------
+
+def interleave_low10(x, y, z):
+    """Independent bit-by-bit Morton interleave of the low 10 bits."""
+    m = 0
+    for b in range(10):
+        m |= ((x >> b) & 1) << (3 * b)
+        m |= ((y >> b) & 1) << (3 * b + 1)
+        m |= ((z >> b) & 1) << (3 * b + 2)
+    return m
+
+
+def test_expand_bits_known_answers(oracle):
+    L = oracle.lib()
+    assert L.pbfo_expand_bits(0) == 0
+    assert L.pbfo_expand_bits(1) == 1
+    assert L.pbfo_expand_bits(2) == 8
+    assert L.pbfo_expand_bits(3) == 9
+    assert L.pbfo_expand_bits(1023) == 0x09249249
+    assert L.pbfo_cell_hash(1, 0, 0, 1 << 30) == 1
+    assert L.pbfo_cell_hash(0, 1, 0, 1 << 30) == 2
+    assert L.pbfo_cell_hash(0, 0, 1, 1 << 30) == 4
+    assert L.pbfo_cell_hash(1, 1, 1, 1 << 30) == 7
+    # x=2 -> bit 3, y=3 -> bits 1,4, z=1 -> bit 2
+    assert L.pbfo_cell_hash(2, 3, 1, 1 << 30) == (1 << 3) | (1 << 1) | (1 << 4) | (1 << 2)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_morton_matches_interleave(oracle, seed):
+    L = oracle.lib()
+    rng = np.random.default_rng(seed)
+    for _ in range(2000):
+        x, y, z = (int(a) for a in rng.integers(-256, 1024, size=3))
+        want = interleave_low10(x & 1023, y & 1023, z & 1023)
+        assert L.pbfo_cell_hash(x, y, z, 1 << 30) == want
+        gbs = int(rng.integers(1, 1 << 22))
+        assert L.pbfo_cell_hash(x, y, z, gbs) == want % gbs
+
+
+def test_expand_bits_below_minus_256(oracle):
+    # For x < -256 the OR with x's own sign bits 24..25 leaks in: the spread
+    # value is that of (x & 0xFF) | 0x300, not of the low 10 bits.
+    L = oracle.lib()
+    for x in (-257, -300, -512, -1024, -100000):
+        assert L.pbfo_expand_bits(x) == interleave_low10((x & 0xFF) | 0x300, 0, 0)
+
+
+def test_cell_coord_truncates_toward_zero(oracle):
+    L = oracle.lib()
+    assert L.pbfo_cell_coord(0.99) == 0
+    assert L.pbfo_cell_coord(-0.99) == 0
+    assert L.pbfo_cell_coord(-1.0) == -1
+    assert L.pbfo_cell_coord(2.5) == 2
+    assert L.pbfo_cell_coord(float("nan")) == 0
+    assert L.pbfo_cell_coord(1e30) == 1 << 24
+
+
+# --- kernel constants (Simulation.cpp:179-180) --------------------------------
+
+@pytest.mark.parametrize("h", [0.1, 0.05, 2.0, 0.0625])
+def test_kernel_factors_six_digits(oracle, h):
+    p6, sp = oracle.kernel_factors(h)
+    hd = float(np.float32(h))
+    exact6 = 315.0 / (64.0 * math.pi * hd ** 9)
+    exacts = 45.0 / (math.pi * hd ** 6)
+    # the reference compiles the 6-significant-digit decimal
+    assert p6 == float(np.float32(float("%.6g" % exact6)))
+    assert sp == float(np.float32(float("%.6g" % exacts)))
+    assert abs(p6 / exact6 - 1) < 6e-6 and abs(sp / exacts - 1) < 6e-6
+
+
+def test_kernel_factors_survey_values(oracle):
+    # 7-digit values of 315/(64 pi h^9) and 45/(pi h^6); the reference keeps 6
+    for h, want6, wants in ((0.1, 1.566681e9, 1.432394e7), (0.05, 8.021409e11, 9.167325e8)):
+        p6, sp = oracle.kernel_factors(h)
+        assert abs(p6 - want6) / want6 < 5e-6
+        assert abs(sp - wants) / wants < 5e-6
+
+
+# --- truncation rule (compute_scaling.cl:50, compute_delta.cl:100) ------------
+
+def test_circle_skip_rule_equals_rational_form(oracle):
+    """fp32 processed/total > 0.6 (0.5) equals 5p > 3t (2p > t) for t <= 250.
+
+    The GPU kernels use the rational form; this pins it to the literal rule."""
+    L = oracle.lib()
+    p = np.arange(251)[:, None].astype(np.float32)
+    t = np.arange(251)[None, :].astype(np.float32)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        q = p / t
+    pi = np.arange(251)[:, None]
+    ti = np.arange(251)[None, :]
+    assert np.array_equal(q > np.float32(0.6), 5 * pi > 3 * ti)
+    assert np.array_equal(q > np.float32(0.5), 2 * pi > ti)
+    for pp, tt in [(3, 5), (0, 0), (4, 6), (150, 250), (151, 250), (125, 250), (126, 250)]:
+        assert L.pbfo_skip_circle(pp, tt, 0.6) == int(5 * pp > 3 * tt)
+        assert L.pbfo_skip_circle(pp, tt, 0.5) == int(2 * pp > tt)
+
+
+# --- sine of the wall noise ----------------------------------------------------
+
+def test_noise_sine_accuracy(oracle):
+    L = oracle.lib()
+    xs = np.linspace(-800.0, 800.0, 20001).astype(np.float32)
+    got = np.array([L.pbfo_sinf(float(x)) for x in xs])
+    err = np.max(np.abs(got - np.sin(xs.astype(np.float64))))
+    assert err < 2e-6, err
+    assert L.pbfo_sinf(float("nan")) == 0.0
+
+
+# --- sort, cells ----------------------------------------------------------------
+
+def test_stable_sort_matches_numpy(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(4)
+    for n, hi in [(1, 5), (1000, 7), (50000, 1 << 22), (3001, 1 << 13)]:
+        keys = rng.integers(0, hi, size=n).astype(np.uint32)
+        ks = np.zeros(n, np.uint32)
+        perm = np.zeros(n, np.uint32)
+        L.pbfo_stable_sort(po.ptr(keys), n, po.ptr(ks), po.ptr(perm))
+        want = np.argsort(keys, kind="stable")
+        assert np.array_equal(perm, want.astype(np.uint32))
+        assert np.array_equal(ks, keys[want])
+
+
+def test_cells_brute_force(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(5)
+    gbs = 512
+    for n in (1, 2, 1000):
+        ks = np.sort(rng.integers(0, gbs, size=n)).astype(np.uint32)
+        cells = np.full(2 * gbs, 0xFFFFFFFF, np.uint32)
+        L.pbfo_cells(po.ptr(ks), n, po.ptr(cells))
+        for k in range(gbs):
+            idx = np.nonzero(ks == k)[0]
+            if len(idx) == 0:
+                assert cells[2 * k] == 0xFFFFFFFF and cells[2 * k + 1] == 0xFFFFFFFF
+            else:
+                assert cells[2 * k] == idx[0] and cells[2 * k + 1] == idx[-1]
+
+
+# --- friends lists ----------------------------------------------------------------
+
+def _friends_of(fr, n, i):
+    counts = fr[: 5 * n].reshape(5, n)[:, i]
+    idx = fr[5 * n:].reshape(5, 50, n)
+    return [list(idx[q, : counts[q], i]) for q in range(5)]
+
+
+def _structure(scene):
+    L = po.lib()
+    c = po.cfg_from_params(scene.params, scene.options)
+    n = scene.n
+    x = scene.x.copy()
+    v = scene.v.copy()
+    xs = np.zeros_like(x)
+    L.pbfo_predict(ctypes.byref(c), 0, po.ptr(x), po.ptr(v), po.ptr(xs))
+    keys = np.zeros(n, np.uint32)
+    L.pbfo_keys(ctypes.byref(c), po.ptr(xs), po.ptr(keys))
+    ks = np.zeros(n, np.uint32)
+    perm = np.zeros(n, np.uint32)
+    L.pbfo_stable_sort(po.ptr(keys), n, po.ptr(ks), po.ptr(perm))
+    xs = np.ascontiguousarray(xs[perm])
+    cells = np.full(2 * c.grid_buf_size, 0xFFFFFFFF, np.uint32)
+    L.pbfo_cells(po.ptr(ks), n, po.ptr(cells))
+    fr = np.zeros(n * 255, np.uint32)
+    L.pbfo_friends(ctypes.byref(c), po.ptr(xs), po.ptr(cells), po.ptr(fr))
+    return c, xs, ks, cells, fr
+
+
+def test_friends_equal_brute_force_and_order(oracle):
+    """Friend set == {j != i : |r|^2 < h^2}; circle rule; scan order."""
+    scene = pbf_amd.dam_break(12, 10, 9, 0.1, (1.2, 1.2, 1.2), seed=21)
+    c, xs, ks, cells, fr = _structure(scene)
+    n = scene.n
+    h = np.float32(c.h)
+    h2 = np.float32(h * h)
+    p = xs[:, :3]
+    rng = np.random.default_rng(1)
+    for i in rng.choice(n, 120, replace=False):
+        r = p[i] - p
+        r2 = (r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1]) + r[:, 2] * r[:, 2]
+        want = set(np.nonzero((r2 < h2) & (np.arange(n) != i))[0].tolist())
+        got = _friends_of(fr, n, i)
+        flat = [j for circ in got for j in circ]
+        assert len(flat) == len(set(flat)) and set(flat) == want
+        ci = np.trunc(p[i] / h).astype(np.int64)
+        for q, circ in enumerate(got):
+            prev = None
+            for j in circ:
+                rl = float(np.sqrt(np.float32(r2[j])))
+                a = max(0.0, (rl - 0.3 * float(h)) / (0.7 * float(h)))
+                assert min(int(a ** 3 * 5), 4) == q or abs(a ** 3 * 5 - round(a ** 3 * 5)) < 1e-4
+                cj = np.trunc(p[j] / h).astype(np.int64) - ci
+                order = ((int(cj[0]) + 1) * 9 + (int(cj[1]) + 1) * 3 + (int(cj[2]) + 1), int(j))
+                assert prev is None or order > prev, "friends not in x,y,z cell scan order"
+                prev = order
+
+
+def test_rest_lattice_neighbours_and_density(oracle):
+    """Exact lattice at spacing h/2 (h = 2^-3, all coordinates exact)."""
+    h = 0.125
+    n1 = 9
+    g = (np.arange(n1) + 2) * (h / 2)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    x = np.zeros((n1 ** 3, 4), np.float32)
+    x[:, 0], x[:, 1], x[:, 2] = X.ravel(), Y.ravel(), Z.ravel()
+    p = pbf_amd.physical_params(len(x), h, (0, 0, 0), (2.0, 2.0, 2.0))
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(walls=False), x, np.zeros_like(x))
+    L = po.lib()
+    c = po.cfg_from_params(scene.params, scene.options)
+    n = scene.n
+    # the lattice as-is (no predict), sorted by key
+    keys = np.zeros(n, np.uint32)
+    L.pbfo_keys(ctypes.byref(c), po.ptr(x), po.ptr(keys))
+    ks = np.zeros(n, np.uint32)
+    perm = np.zeros(n, np.uint32)
+    L.pbfo_stable_sort(po.ptr(keys), n, po.ptr(ks), po.ptr(perm))
+    xs = np.ascontiguousarray(x[perm])
+    cells = np.full(2 * c.grid_buf_size, 0xFFFFFFFF, np.uint32)
+    L.pbfo_cells(po.ptr(ks), n, po.ptr(cells))
+    fr = np.zeros(n * 255, np.uint32)
+    L.pbfo_friends(ctypes.byref(c), po.ptr(xs), po.ptr(cells), po.ptr(fr))
+    rho = np.zeros(n, np.float32)
+    lam = np.zeros(n, np.float32)
+    L.pbfo_scaling(ctypes.byref(c), po.ptr(xs), po.ptr(fr), po.ptr(rho), po.ptr(lam))
+    centre = np.nonzero(np.all(xs[:, :3] == np.float32((4 + 2) * h / 2), axis=1))[0][0]
+    counts = fr[: 5 * n].reshape(5, n)[:, centre]
+    assert list(counts) == [18, 0, 8, 0, 0]
+    poly6, spiky = oracle.kernel_factors(h)
+    # circle 0 only (18/26 > 0.6 skips the rest): 6 at h/2, 12 at h/sqrt(2)
+    rho_want = poly6 * (6 * (h * h * 0.75) ** 3 + 12 * (h * h * 0.5) ** 3)
+    assert abs(rho[centre] / rho_want - 1) < 1e-6
+    assert abs(rho[centre] * h ** 3 - 6.3157) < 2e-4
+    g1 = spiky * (h / 2) ** 2
+    g2 = spiky * (h - h / math.sqrt(2)) ** 2
+    gsum = 6 * g1 * g1 + 12 * g2 * g2            # sum grad is zero by symmetry
+    rd = float(scene.params.restDensity)
+    lam_want = -(rho_want / rd - 1) / (gsum / (rd * rd) + float(scene.params.epsilon))
+    assert abs(lam[centre] / lam_want - 1) < 1e-5
+
+
+def test_two_particle_delta_closed_form(oracle):
+    """Delta p of an isolated pair against the closed form of compute_delta.cl."""
+    h = 0.1
+    d = 0.037
+    x = np.zeros((2, 4), np.float32)
+    x[0, :3] = (0.5, 0.5, 0.5)
+    x[1, :3] = (0.5 + d, 0.5, 0.5)
+    p = pbf_amd.physical_params(2, h, (0, 0, 0), (1, 1, 1))
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(walls=False), x, np.zeros_like(x))
+    c = po.cfg_from_params(scene.params, scene.options)
+    L = po.lib()
+    fr = np.zeros(2 * 255, np.uint32)
+    fr[0:2] = 1                                    # one friend each, circle 0
+    fr[5 * 2 + 0] = 1
+    fr[5 * 2 + 1] = 0
+    rho = np.zeros(2, np.float32)
+    lam = np.zeros(2, np.float32)
+    L.pbfo_scaling(ctypes.byref(c), po.ptr(x), po.ptr(fr), po.ptr(rho), po.ptr(lam))
+    poly6, spiky = oracle.kernel_factors(h)
+    rd = float(p.restDensity)
+    rho_want = poly6 * (h * h - d * d) ** 3
+    g = spiky * (h - d) ** 2
+    lam_want = -(rho_want / rd - 1) / (2 * g * g / (rd * rd) + float(p.epsilon))
+    assert abs(rho[0] / rho_want - 1) < 2e-6 and abs(lam[0] / lam_want - 1) < 2e-6
+    xs = x.copy()
+    xs[:, 3] = lam
+    delta = np.zeros_like(x)
+    L.pbfo_delta(ctypes.byref(c), po.ptr(x), po.ptr(xs), po.ptr(fr), po.ptr(delta))
+    wq = (h * h - (0.2 * h) ** 2) ** 3
+    scorr = float(p.surfaceTenstionK) * ((h * h - d * d) ** 3 / wq) ** 4
+    dx_want = -spiky * (2 * lam_want + scorr) * (h - d) ** 2 * (-1.0) / rd   # r = p0 - p1 = -d x
+    assert abs(delta[0, 0] / dx_want - 1) < 1e-5
+    assert abs(delta[1, 0] / -dx_want - 1) < 1e-5
+    assert delta[0, 1] == 0 and delta[0, 2] == 0
+
+
+def test_walls_single_particle(oracle):
+    """Box planes: a particle pushed below the floor lands on floor + edge."""
+    h = 0.1
+    x = np.zeros((1, 4), np.float32)
+    x[0, :3] = (0.5, 0.004, 0.5)                    # previous position: inside
+    xs = x.copy()
+    xs[0, 1] = -0.02                               # predicted: through the floor
+    p = pbf_amd.physical_params(1, h, (0, 0, 0), (1, 1, 1))
+    c = po.cfg_from_params(p, pbf_amd.default_options())
+    fr = np.zeros(255, np.uint32)
+    delta = np.zeros_like(x)
+    L = po.lib()
+    L.pbfo_delta(ctypes.byref(c), po.ptr(x), po.ptr(xs), po.ptr(fr), po.ptr(delta))
+    f = xs[0, :3] + delta[0, :3]
+    s = [L.pbfo_sinf(float(np.float32(np.float32(v * 5.0) / np.float32(h)))) for v in (0.5, -0.02, 0.5)]
+    edge = np.float32(np.float32(np.float32(np.float32(np.float32(3.0 + s[0]) + s[1]) + s[2]) * np.float32(h)) * np.float32(0.03))
+    # the stage stores delta = future - x*, applied as x* + delta
+    assert f[1] == np.float32(xs[0, 1] + np.float32(edge - xs[0, 1]))
+    assert abs(float(f[1]) - float(edge)) <= 1e-9
+    assert f[0] == np.float32(0.5) and f[2] == np.float32(0.5)
+    assert 0 <= edge <= 0.18 * h
+    # a particle that started outside the wall is not caught (force-zone test)
+    x[0, 1] = -0.5
+    L.pbfo_delta(ctypes.byref(c), po.ptr(x), po.ptr(xs), po.ptr(fr), po.ptr(delta))
+    assert xs[0, 1] + delta[0, 1] == np.float32(-0.02)
+
+
+def test_step_thread_count_invariance(oracle, monkeypatch):
+    """Bitwise identical results whatever the OpenMP thread count."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path[:0]=['position-based-fluids_amd','oracle'];"
+        "import numpy as np, pbf_amd, pbf_oracle as po;"
+        "s=pbf_amd.config('dam4k',seed=2); o=po.OracleSim(po.cfg_from_params(s.params,s.options));"
+        "o.set_state(s.x,s.v); [o.step() for _ in range(5)];"
+        "sys.stdout.write(o.get_state()['x'].tobytes().hex()[:4000] + str(hash(o.get_state()['v'].tobytes())))"
+    )
+    from pathlib import Path
+    root = Path(__file__).resolve().parent.parent
+    outs = []
+    for threads in ("1", "4"):
+        env = dict(__import__("os").environ, OMP_NUM_THREADS=threads, PYTHONHASHSEED="0")
+        outs.append(subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True,
+                                   text=True, check=True).stdout)
+    assert outs[0] == outs[1]
+
+
+def test_dam_break_stays_in_box(oracle):
+    scene = pbf_amd.config("dam4k", seed=8)
+    o = po.OracleSim(po.cfg_from_params(scene.params, scene.options))
+    o.set_state(scene.x, scene.v)
+    for _ in range(30):
+        o.step()
+    st = o.get_state()
+    x = st["x"][:, :3]
+    assert np.all(np.isfinite(x))
+    assert np.all(x >= 0.0) and np.all(x <= 1.6)
+    speed = np.linalg.norm(st["v"][:, :3], axis=1)
+    assert speed.max() < 20.0

@@ -32,6 +32,8 @@ extern "C" {
 #define PBF_FLAG_SORT_TIMEOUT 1u   /* radix-sort look-back wait hit its bound    */
 #define PBF_FLAG_FRIENDS_OVERFLOW 2u /* a circle held 50 friends; extras dropped  */
 #define PBF_FLAG_NAN 4u            /* non-finite predicted position seen        */
+#define PBF_FLAG_INTERNAL 8u       /* an internal bounds check failed (a result
+                                      of this step is invalid)                 */
 
 typedef struct pbf_ctx pbf_ctx;
 

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kThreads) void k_predict_hash(const float4* __restr
             if (reset_old) {
                 // reset_grid.cl: empty the cells of the previous step
                 const uint32_t ok = old_keys[i];
-                if (i == 0 || old_keys[i - 1] != ok) cells[ok] = make_uint2(kEmpty, kEmpty);
+                if ((i == 0 || old_keys[i - 1] != ok) && ok < c.gbs) cells[ok] = make_uint2(kEmpty, kEmpty);
             }
             bad |= !(fabsf(p.x) <= 3.4e38f && fabsf(p.y) <= 3.4e38f && fabsf(p.z) <= 3.4e38f);
         }
@@ -94,10 +94,15 @@ __global__ __launch_bounds__(kThreads) void k_predict_only(const float4* __restr
 // hashed; v itself is not permuted (it is rewritten before it is read).
 __global__ __launch_bounds__(kThreads) void k_gather(const uint32_t* __restrict__ perm, const float4* __restrict__ x,
                                                     const float4* __restrict__ v, float4* __restrict__ x_out,
-                                                    float4* __restrict__ xs_out, uint32_t pause, StepConsts c) {
+                                                    float4* __restrict__ xs_out, uint32_t pause, StepConsts c,
+                                                    uint32_t* err) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= c.n) return;
-    const uint32_t p = perm[i];
+    uint32_t p = perm[i];
+    if (p >= c.n) {  // only after a failed sort (flagged there)
+        atomicOr(err, (uint32_t)PBF_FLAG_INTERNAL);
+        p = i;
+    }
     const float4 xp = x[p];
     x_out[i] = xp;
     xs_out[i] = predicted_from(xp, v[p], c, pause);
@@ -105,10 +110,14 @@ __global__ __launch_bounds__(kThreads) void k_gather(const uint32_t* __restrict_
 
 // update_cells.cl:1-26 (the last particle always closes its cell).
 __global__ __launch_bounds__(kThreads) void k_cells(const uint32_t* __restrict__ ks, uint32_t* __restrict__ cells,
-                                                   uint32_t n) {
+                                                   uint32_t n, uint32_t gbs, uint32_t* err) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = ks[i];
+    if (k >= gbs || (i > 0 && ks[i - 1] >= gbs)) {  // only after a failed sort
+        atomicOr(err, (uint32_t)PBF_FLAG_INTERNAL);
+        return;
+    }
     if (i == 0) {
         cells[2 * (size_t)k] = 0;
     } else {
@@ -368,7 +377,7 @@ hipError_t launch_predict_only(pbf_ctx* ctx, uint32_t pause) {
 hipError_t launch_gather(pbf_ctx* ctx) {
     track_begin(ctx, "sortParticles");
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream, ctx->perm,
-                       ctx->x, ctx->v, ctx->x_alt, ctx->xs[0], ctx->pause_now, ctx->c);
+                       ctx->x, ctx->v, ctx->x_alt, ctx->xs[0], ctx->pause_now, ctx->c, ctx->err);
     track_end(ctx);
     float4* t = ctx->x;
     ctx->x = ctx->x_alt;
@@ -380,7 +389,7 @@ hipError_t launch_gather(pbf_ctx* ctx) {
 hipError_t launch_cells(pbf_ctx* ctx) {
     track_begin(ctx, "updateCells");
     hipLaunchKernelGGL(k_cells, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                       ctx->keys_sorted, ctx->cells, ctx->n);
+                       ctx->keys_sorted, ctx->cells, ctx->n, ctx->c.gbs, ctx->err);
     track_end(ctx);
     return hipGetLastError();
 }

@@ -141,8 +141,12 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint32_t* __res
         if (idx < n) {
             const uint32_t dg = (key[k] >> shift) & (kRadix - 1);
             const uint32_t dst = s_off[dg] + s_cnt[w][dg] + rank[k];
-            kout[dst] = key[k];
-            vout[dst] = val[k];
+            if (dst < n) {
+                kout[dst] = key[k];
+                vout[dst] = val[k];
+            } else {
+                atomicOr(err, (uint32_t)PBF_FLAG_INTERNAL);
+            }
         }
     }
 }

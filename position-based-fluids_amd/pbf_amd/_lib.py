@@ -20,6 +20,7 @@ PBF_E_STATE = -4
 PBF_FLAG_SORT_TIMEOUT = 1
 PBF_FLAG_FRIENDS_OVERFLOW = 2
 PBF_FLAG_NAN = 4
+PBF_FLAG_INTERNAL = 8
 
 
 class PbfParams(ctypes.Structure):

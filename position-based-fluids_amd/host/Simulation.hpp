@@ -1,0 +1,94 @@
+// Simulation.hpp -- C++ drop-in for the reference's Simulation class
+// (src/Simulation.hpp:28-151) on top of the pbf C ABI.
+//
+// Same public surface for the caller (Runner::run, src/Runner.cpp:43-132):
+// InitBuffers / InitCells / InitKernels / Step, the bPauseSim /
+// bReadFriendsList / fWavePos flags, PerfData trackers, and the ping/pong
+// position-buffer swap the renderer draws from.  OpenCL objects are gone:
+// the constructor takes no cl::Context, buffers live in HBM, and the GL
+// hand-off is a host staging copy (CopyRenderBuffers) until a GL context is
+// available to register the buffers with HIP.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pbf/pbf.h"
+
+namespace pbf_host {
+
+// Macro used for the end of cell list (src/Simulation.hpp:22)
+static const int END_OF_CELL_LIST = -1;
+
+// A global parameter object and loader (src/ParamUtils.hpp:10-13).
+extern pbf_params Params;
+void LoadParameters(const std::string& text);
+
+// OCLPerfMon::Trackers equivalent (src/OCLPerfMon.h).
+struct PerfTracker {
+    std::string eventName;
+    double total_time = 0.0;  // EMA, ms
+    double last_time = 0.0;   // ms
+};
+struct PerfMon {
+    std::vector<PerfTracker> Trackers;
+};
+
+class Simulation {
+public:
+    // The reference takes the CL context and device; the HIP context is the
+    // current device of the calling thread.
+    Simulation();
+    explicit Simulation(const pbf_options& options);
+    ~Simulation();
+    Simulation(const Simulation&) = delete;
+    Simulation& operator=(const Simulation&) = delete;
+
+    // src/Simulation.cpp:206-244: allocates buffers and places the initial
+    // particle block (CreateParticles, :71-101).
+    void InitBuffers();
+    // src/Simulation.cpp:246-255: cells to END_OF_CELL_LIST, friends to 0.
+    void InitCells();
+    // src/Simulation.cpp:129-204: nothing to compile at run time (the HIP
+    // code objects are built ahead of time); returns true.
+    bool InitKernels();
+    // src/Simulation.cpp:554-621
+    void Step();
+
+    // Replace the particle state (float4 x and v, particle-major).
+    void SetState(const std::vector<float>& x4, const std::vector<float>& v4);
+    // Host staging of the two render inputs: the positions VBO (w = speed)
+    // and the 2048-wide RGBA32F texture contents (x*, w = density).
+    void CopyRenderBuffers(std::vector<float>* positions4, std::vector<float>* texture4) const;
+    void GetState(std::vector<float>* x4, std::vector<float>* v4) const;
+    // Collect per-stage timings into PerfData (OCLPerfMon::UpdateTimings).
+    void UpdateTimings();
+
+    pbf_ctx* context() const { return mCtx; }
+
+    // Rendering state (src/Simulation.hpp:147-150)
+    bool bPauseSim = false;
+    bool bReadFriendsList = false;
+    bool bDumpParticlesData = false;
+    float fWavePos = 0.0f;
+
+    // The renderer draws mSharedPingBufferID; Step() swaps the pair after a
+    // sort as the reference does (src/Simulation.cpp:535).
+    uint32_t mSharedPingBufferID = 0;
+    uint32_t mSharedPongBufferID = 0;
+    uint32_t mSharedParticlesPos = 0;
+
+    PerfMon PerfData;
+
+private:
+    void CreateParticles();
+    void Check(int rc, const char* what) const;
+
+    pbf_options mOptions{};
+    pbf_ctx* mCtx = nullptr;
+};
+
+}  // namespace pbf_host

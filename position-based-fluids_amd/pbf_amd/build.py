@@ -18,6 +18,10 @@ CSRC = PKG_ROOT / "csrc"
 BUILD = PKG_ROOT / "_build"
 LIB_DIR = PKG_ROOT / "lib"
 LIB_PATH = LIB_DIR / "libpbf_amd.so"
+HOST = PKG_ROOT / "host"
+SIM_LIB_PATH = LIB_DIR / "libpbf_sim.so"
+BIN_DIR = PKG_ROOT / "bin"
+RUN_PATH = BIN_DIR / "pbf_run"
 
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -71,9 +75,31 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB_PATH)
+    build_host()
     if verbose:
-        print(f"built {LIB_PATH}")
+        print(f"built {LIB_PATH}, {SIM_LIB_PATH}, {RUN_PATH}")
     return LIB_PATH
+
+
+def build_host() -> None:
+    """C++ Simulation drop-in (host only, g++) over the C ABI, and pbf_run."""
+    BIN_DIR.mkdir(parents=True, exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    common = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{REPO_ROOT / 'include'}"]
+    tmp = SIM_LIB_PATH.with_suffix(".so.tmp")
+    cmd = [cxx, *common, "-shared", str(HOST / "Simulation.cpp"), "-o", str(tmp), f"-L{LIB_DIR}", "-lpbf_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, SIM_LIB_PATH)
+    tmp = RUN_PATH.with_suffix(".tmp")
+    cmd = [cxx, *common, str(HOST / "pbf_run.cpp"), "-o", str(tmp), f"-L{LIB_DIR}", "-lpbf_sim", "-lpbf_amd",
+           "-Wl,-rpath,$ORIGIN/../lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"pbf_run build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, RUN_PATH)
 
 
 if __name__ == "__main__":

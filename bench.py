@@ -20,6 +20,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "position-based-fluids_amd"))
 
@@ -87,6 +89,59 @@ def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
     }
 
 
+def batched_gain(steps: int, warmup: int, seed: int, tune: bool) -> dict:
+    """64-scene ensemble: one launch per stage (graph-replayed substeps)
+    against looping the single-scene path over the same scenes."""
+    import pbf_amd
+    scenes = pbf_amd.ensemble(64, seed=seed)
+    total = sum(s.n for s in scenes)
+    x0 = np.concatenate([s.x for s in scenes])
+    v0 = np.concatenate([s.v for s in scenes])
+
+    batch = pbf_amd.BatchSimulation([s.params for s in scenes], [s.options for s in scenes])
+    batch.set_state(x0, v0)
+    batch.run(warmup)
+    sizes = batch.autotune(2) if tune else None
+    batch.set_graph(True)
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.run(steps)
+    batch.sync()
+    t_batch = time.perf_counter() - t0
+    batch.close()
+
+    singles = []
+    for s in scenes:
+        sim = pbf_amd.Simulation(s.params, s.options)
+        sim.set_state(s.x, s.v)
+        if sizes:
+            sim.set_workgroup_sizes(sizes)
+        singles.append(sim)
+    for _ in range(warmup):
+        for sim in singles:
+            sim.Step()
+    for sim in singles:
+        sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sim in singles:
+            sim.Step()
+    for sim in singles:
+        sim.sync()
+    t_loop = time.perf_counter() - t0
+    for sim in singles:
+        sim.close()
+    return {
+        "scenes": len(scenes),
+        "particles": total,
+        "steps": steps,
+        "batched_particle_substeps_per_s": total * steps / t_batch,
+        "looped_particle_substeps_per_s": total * steps / t_loop,
+        "gain": t_loop / t_batch,
+        "workgroup_sizes": sizes,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +150,11 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default workgroup sizes")
+    ap.add_argument("--batched-steps", type=int, default=30)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain HBM copy of MIB MiB (calibration under rocprofv3) and print GB/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -102,12 +162,19 @@ def main() -> int:
         return 2
 
     import pbf_amd
+    if args.copy_probe:
+        from pbf_amd._lib import copy_probe
+        print(json.dumps({"copy_probe_mib": args.copy_probe, "reps": 10,
+                          "gbps": copy_probe(args.copy_probe << 20, 10)}))
+        return 0
     scene = pbf_amd.config("dam1m", seed=args.seed)
     n = scene.n
     iters = int(scene.params.simIterations)
     sim = pbf_amd.Simulation(scene.params, scene.options)
     sim.set_state(scene.x, scene.v)
     sim.run(args.warmup)
+    # tuned on the warmed-up state, which autotune restores afterwards
+    sizes = sim.autotune(2) if not args.no_autotune else sim.stats()["workgroup_size"]
     sim.sync()
     sim.set_profiling(True)
     sim.sync()
@@ -129,7 +196,11 @@ def main() -> int:
     fam_ms: dict[str, float] = {}
     for name, t in timings.items():
         fam_ms[family(name)] = fam_ms.get(family(name), 0.0) + t["sum_ms"]
-    dom = max(fam_ms, key=fam_ms.get)
+    # dominant kernel: the family with the most time among those with an
+    # algorithmic byte figure (the friends build moves index structure only)
+    measured = {k: v for k, v in fam_ms.items() if kernel_bytes_per_particle(k, iters, passes) is not None}
+    dom = max(measured, key=measured.get)
+    top = max(fam_ms, key=fam_ms.get)
     launches = [(name, t) for name, t in timings.items() if family(name) == dom]
     tot_ms = sum(t["sum_ms"] for _, t in launches)
     tot_calls = sum(t["calls"] for _, t in launches)
@@ -160,6 +231,7 @@ def main() -> int:
             "grid_buf_size": int(scene.params.gridBufSize),
             "sort_passes": passes,
             "substeps_before_timing": args.warmup,
+            "workgroup_sizes": sizes,
         },
         "roofline": {
             "bound": "hbm",
@@ -171,11 +243,14 @@ def main() -> int:
             "traffic": None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
+            "top_kernel_overall": top,
         },
         "stage_ms_per_step": step_ms,
         "error_flags": flags,
         "dropped_friends": dropped,
     }
+    if not args.no_batched:
+        out["batched"] = batched_gain(args.batched_steps, 5, args.seed, not args.no_autotune)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget)
     print(json.dumps(out))

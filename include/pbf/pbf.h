@@ -64,6 +64,17 @@ int pbf_load_parameters(const char *text, pbf_params *p);
  * buffer for params->particleCount particles. opt may be NULL. */
 int pbf_create(const pbf_params *params, const pbf_options *opt, pbf_ctx **out);
 
+/* A batch of independent scenes stepped together: params[s] and opts[s]
+ * (opts may be NULL) describe scene s; all scenes share simIterations.
+ * Particles of scene s occupy [offset[s], offset[s+1]) of every array
+ * (pbf_scene_offsets), friend and permutation indices are global, cell keys
+ * are offset by the preceding scenes' gridBufSize.  Each stage of a step is
+ * one launch over all scenes.  No counterpart in the reference, which runs
+ * one scene per Simulation. */
+int pbf_create_batch(const pbf_params *params, const pbf_options *opts, uint32_t scenes, pbf_ctx **out);
+int pbf_scene_count(pbf_ctx *ctx);
+int pbf_scene_offsets(pbf_ctx *ctx, uint32_t *offsets /* scenes + 1 */);
+
 /* Initial particle state, the role of Simulation::CreateParticles
  * (src/Simulation.cpp:71-101): n must equal particleCount; v may be NULL
  * (zero velocities). */
@@ -76,6 +87,10 @@ int pbf_step(pbf_ctx *ctx, uint32_t pause);
 /* steps x Simulation::Step, as Runner::run does per frame
  * (src/Runner.cpp:124-132). */
 int pbf_run(pbf_ctx *ctx, uint32_t steps, uint32_t pause);
+
+/* Replays each substep as an instantiated HIP graph (captured on first use
+ * for every variant of the buffer rotation).  Ignored while profiling. */
+int pbf_set_graph(pbf_ctx *ctx, int enable);
 
 /* Blocks until all queued work of the context is done. */
 int pbf_sync(pbf_ctx *ctx);
@@ -120,10 +135,47 @@ typedef struct pbf_timing {
 } pbf_timing;
 int pbf_set_profiling(pbf_ctx *ctx, int enable);
 int pbf_timings(pbf_ctx *ctx, pbf_timing *out, int cap, int *count);
+int pbf_reset_timings(pbf_ctx *ctx);
+
+/* Launch geometry.  Families: 0 friends build, 1 lambda, 2 delta,
+ * 3 viscosity, 4 vorticity; sizes from {64, 128, 256, 512}.  The friends
+ * build stages one workgroup's neighbourhood (about size/8 cells) in LDS, so
+ * its size is also the number of cells per tile. */
+#define PBF_FAMILIES 5
+int pbf_set_workgroup_sizes(pbf_ctx *ctx, const uint32_t *sizes /* PBF_FAMILIES */);
+/* Friends build variant: 1 stages each tile's 27-cell neighbourhood in LDS,
+ * 0 reads candidates from global memory (L2).  Both give the same lists. */
+int pbf_set_friends_mode(pbf_ctx *ctx, int staged);
+/* Autotuner: times every candidate size for each family (and both friends
+ * build variants) over `steps`
+ * substeps from the current state (which is restored afterwards), keeps the
+ * fastest, and returns the chosen sizes in `chosen` (may be NULL). */
+int pbf_autotune(pbf_ctx *ctx, uint32_t steps, uint32_t *chosen /* PBF_FAMILIES */);
+
+typedef struct pbf_stats_t {
+    uint32_t particles, scenes, total_cells;
+    uint32_t sort_passes, sort_tiles;
+    uint32_t unstaged_tiles;      /* friends tiles that read global memory */
+    uint32_t workgroup_size[PBF_FAMILIES];
+    uint32_t friends_staged;      /* friends build variant (pbf_set_friends_mode) */
+    uint32_t reserved[7];
+} pbf_stats_t;
+int pbf_stats(pbf_ctx *ctx, pbf_stats_t *out);
 
 /* Accumulated PBF_FLAG_* bits and the number of dropped friends; reset
  * clears them.  Synchronises. */
 int pbf_error_flags(pbf_ctx *ctx, uint32_t *flags, uint64_t *dropped_friends, int reset);
+
+/* The r^2 at which the friend circle of build_friends_list.cl:75-76 reaches
+ * 1, 2, 3, 4 for smoothing length h (+inf if never in range): the kernels
+ * classify a friend by comparing its binary32 r^2 against these. */
+int pbf_friend_circle_thresholds(float h, float *t4);
+
+/* Plain HBM copy of `bytes` (float4 loads/stores, 2048 workgroups), `reps`
+ * times; *gbps receives read+write bytes per second.  No counterpart in the
+ * reference: it calibrates the roofline (sustained copy bandwidth) and the
+ * rocprofv3 FETCH_SIZE / WRITE_SIZE counters (kernel k_copy_probe). */
+int pbf_copy_probe(uint64_t bytes, uint32_t reps, double *gbps);
 
 void pbf_destroy(pbf_ctx *ctx);
 

@@ -3,7 +3,9 @@
 //
 // pbf_step enqueues the whole substep on the context's stream with no host
 // synchronisation (the reference blocks six times per step inside its sort,
-// src/Simulation.cpp:488, and once at GL unlock, :551).
+// src/Simulation.cpp:488, and once at GL unlock, :551).  A context holds one
+// scene (pbf_create) or a batch of independent scenes (pbf_create_batch)
+// that every kernel covers in one launch through per-scene offset tables.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -51,17 +53,60 @@ void kernel_factors(float h, float* poly6, float* spiky) {
     *spiky = strtof(buf, nullptr);
 }
 
+// The reference's friend circle of an in-range candidate
+// (build_friends_list.cl:75-76), evaluated in binary32 as the kernels did.
+int circle_of(float r2, float min_r, float span) {
+    float a = (sqrtf(r2) - min_r) / span;
+    a = a > 0.0f ? a : 0.0f;
+    const int c = (int)(a * a * a * 5.0f);
+    return c < 4 ? c : 4;
+}
+
+// Smallest non-negative binary32 r^2 whose circle is >= k (k = 1..4), by
+// bisection over the bit patterns (ordered like the values for r^2 >= 0); a
+// circle no in-range r^2 reaches gets +inf.
+void circle_thresholds(float h2, float min_r, float span, float* t) {
+    uint32_t hb;
+    memcpy(&hb, &h2, 4);
+    for (int k = 1; k <= 4; ++k) {
+        uint32_t lo = 0, hi = hb;   // circle(lo) < k is not assumed; search [0, hb]
+        float hv;
+        memcpy(&hv, &hi, 4);
+        if (circle_of(hv, min_r, span) < k) {
+            t[k - 1] = INFINITY;
+            continue;
+        }
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            float mv;
+            memcpy(&mv, &mid, 4);
+            if (circle_of(mv, min_r, span) >= k) hi = mid; else lo = mid + 1;
+        }
+        memcpy(&t[k - 1], &lo, 4);
+    }
+}
+
 template <class T>
 hipError_t dalloc(T** p, size_t count) {
     return hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1));
 }
 
+void drop_graphs(pbf_ctx* ctx) {
+    for (pbf::GraphEntry& g : ctx->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    ctx->graphs.clear();
+}
+
 void free_all(pbf_ctx* ctx) {
+    drop_graphs(ctx);
     void* ptrs[] = {ctx->x, ctx->x_alt, ctx->xs[0], ctx->xs[1], ctx->v, ctx->v_alt, ctx->omega, ctx->rho,
                     ctx->lambda, ctx->keys_in, ctx->keys_tmp, ctx->keys_sorted, ctx->vals_a, ctx->vals_b,
-                    ctx->perm, ctx->sort.ws, ctx->cells, ctx->nb_cnt, ctx->nb_idx, ctx->err};
+                    ctx->perm, ctx->sort.ws, ctx->cells, ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->err, ctx->sc_d,
+                    ctx->bt_predict.d};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (auto& t : ctx->bt)
+        if (t.d) (void)hipFree(t.d);
     for (pbf::EventPair& e : ctx->events) {
         (void)hipEventDestroy(e.start);
         (void)hipEventDestroy(e.stop);
@@ -70,36 +115,36 @@ void free_all(pbf_ctx* ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
-int validate(const pbf_params* p, pbf_ctx* ctx) {
+int validate(const pbf_params* p, pbf_ctx* ctx, uint32_t scene) {
     if (!p) return fail(ctx, PBF_E_INVALID, "params is NULL");
-    if (p->particleCount >= (1u << 30)) return fail(ctx, PBF_E_INVALID, "particleCount %u >= 2^30", p->particleCount);
-    if (!(p->h > 0.0f) || !isfinite(p->h)) return fail(ctx, PBF_E_INVALID, "h must be positive and finite");
-    if (!(p->timeStep > 0.0f)) return fail(ctx, PBF_E_INVALID, "timeStep must be positive");
+    if (!(p->h > 0.0f) || !isfinite(p->h)) return fail(ctx, PBF_E_INVALID, "scene %u: h must be positive and finite", scene);
+    if (!(p->timeStep > 0.0f)) return fail(ctx, PBF_E_INVALID, "scene %u: timeStep must be positive", scene);
     if (p->simIterations < 1 || p->simIterations > 64)
-        return fail(ctx, PBF_E_INVALID, "simIterations must be in [1, 64] (got %u)", p->simIterations);
+        return fail(ctx, PBF_E_INVALID, "scene %u: simIterations must be in [1, 64] (got %u)", scene, p->simIterations);
     if (p->gridBufSize < 1 || p->gridBufSize > (1u << 30))
-        return fail(ctx, PBF_E_INVALID, "gridBufSize must be in [1, 2^30] (got %u)", p->gridBufSize);
+        return fail(ctx, PBF_E_INVALID, "scene %u: gridBufSize must be in [1, 2^30] (got %u)", scene, p->gridBufSize);
     if (p->friendsCircles != (uint32_t)pbf::kCircles || p->particlesPerCircle != (uint32_t)pbf::kPerCircle)
-        return fail(ctx, PBF_E_INVALID, "friendsCircles/particlesPerCircle must be 5/50 (call pbf_params_finalize)");
-    if (!(p->restDensity != 0.0f)) return fail(ctx, PBF_E_INVALID, "restDensity must be nonzero");
+        return fail(ctx, PBF_E_INVALID,
+                    "scene %u: friendsCircles/particlesPerCircle must be 5/50 (call pbf_params_finalize)", scene);
+    if (!(p->restDensity != 0.0f)) return fail(ctx, PBF_E_INVALID, "scene %u: restDensity must be nonzero", scene);
     return PBF_OK;
 }
 
-pbf::StepConsts make_consts(const pbf_params& P, const pbf_options& o) {
-    pbf::StepConsts c{};
+pbf::SceneConsts make_consts(const pbf_params& P, const pbf_options& o, uint32_t cell_base, uint32_t first) {
+    pbf::SceneConsts c{};
     c.h = P.h;
     c.h2 = P.h_2;
     c.min_r = 0.3f * P.h;            // build_friends_list.cl:32
     c.span = P.h - c.min_r;          // build_friends_list.cl:75
     c.dt = P.timeStep;
     if (o.use_gravity_vector) {
-        c.g[0] = o.gravity[0];
-        c.g[1] = o.gravity[1];
-        c.g[2] = o.gravity[2];
+        c.g0 = o.gravity[0];
+        c.g1 = o.gravity[1];
+        c.g2 = o.gravity[2];
     } else {
-        c.g[0] = 0.0f;               // predict_positions.cl:13
-        c.g[1] = -P.garvity;
-        c.g[2] = 0.0f;
+        c.g0 = 0.0f;                 // predict_positions.cl:13
+        c.g1 = -P.garvity;
+        c.g2 = 0.0f;
     }
     c.rest_density = P.restDensity;
     c.epsilon = P.epsilon;
@@ -111,14 +156,227 @@ pbf::StepConsts make_consts(const pbf_params& P, const pbf_options& o) {
     c.poly6_q = tq * tq * tq;
     c.viscosity = P.viscosityFactor;
     c.vorticity = P.vorticityFactor;
-    c.lo[0] = P.xMin; c.lo[1] = P.yMin; c.lo[2] = P.zMin;
-    c.hi[0] = P.xMax; c.hi[1] = P.yMax; c.hi[2] = P.zMax;
+    c.lo0 = P.xMin; c.lo1 = P.yMin; c.lo2 = P.zMin;
+    c.hi0 = P.xMax; c.hi1 = P.yMax; c.hi2 = P.zMax;
     c.walls = o.walls ? 1u : 0u;
     c.gbs = P.gridBufSize;
-    c.gbs_mask = ((P.gridBufSize & (P.gridBufSize - 1)) == 0) ? P.gridBufSize - 1 : 0u;
-    if (P.gridBufSize == 1) c.gbs_mask = 0;  // m % 1 == 0 handled by the modulo path
-    c.n = P.particleCount;
+    c.gbs_mask = (P.gridBufSize > 1 && (P.gridBufSize & (P.gridBufSize - 1)) == 0) ? P.gridBufSize - 1 : 0u;
+    c.cell_base = cell_base;
+    c.first = first;
+    c.count = P.particleCount;
+    circle_thresholds(c.h2, c.min_r, c.span, c.circ_t);
     return c;
+}
+
+// Block table: one {scene, first, end, 0} per workgroup of `per` particles.
+std::vector<uint4> block_table(const pbf_ctx* ctx, uint32_t per) {
+    std::vector<uint4> t;
+    for (uint32_t s = 0; s < ctx->S; ++s) {
+        const uint32_t first = ctx->sc[s].first, end = first + ctx->sc[s].count;
+        for (uint32_t b = first; b < end; b += per) t.push_back(make_uint4(s, b, std::min(end, b + per), 0u));
+    }
+    return t;
+}
+
+int create_impl(const pbf_params* params, const pbf_options* opts, uint32_t S, pbf_ctx** out) {
+    if (!out) return fail(nullptr, PBF_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!params || S == 0) return fail(nullptr, PBF_E_INVALID, "no scenes");
+    uint64_t n_total = 0, cells_total = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        const int rc = validate(&params[s], nullptr, s);
+        if (rc) return rc;
+        if (params[s].simIterations != params[0].simIterations)
+            return fail(nullptr, PBF_E_INVALID, "scene %u: all scenes of a batch share simIterations", s);
+        n_total += params[s].particleCount;
+        cells_total += params[s].gridBufSize;
+    }
+    if (n_total >= (1u << 29)) return fail(nullptr, PBF_E_INVALID, "%llu particles >= 2^29", (unsigned long long)n_total);
+    if (cells_total > (1u << 30)) return fail(nullptr, PBF_E_INVALID, "total gridBufSize > 2^30");
+
+    pbf_ctx* ctx = new (std::nothrow) pbf_ctx();
+    if (!ctx) return fail(nullptr, PBF_E_NOMEM, "host allocation failed");
+    ctx->S = S;
+    ctx->n = (uint32_t)n_total;
+    ctx->total_cells = (uint32_t)cells_total;
+    ctx->iters = params[0].simIterations;
+    uint32_t first = 0, cell_base = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        ctx->P.push_back(params[s]);
+        pbf_options o;
+        if (opts)
+            o = opts[s];
+        else
+            pbf_default_options(&o);
+        ctx->opt.push_back(o);
+        ctx->sc.push_back(make_consts(params[s], o, cell_base, first));
+        first += params[s].particleCount;
+        cell_base += params[s].gridBufSize;
+    }
+
+    auto bail = [&](int code) {
+        g_create_error = ctx->last_error;
+        free_all(ctx);
+        delete ctx;
+        return code;
+    };
+#define CREATE_TRY(call)                                                                                    \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            const int code_ = e_ == hipErrorOutOfMemory ? PBF_E_NOMEM : PBF_E_HIP;                          \
+            fail(ctx, code_, "%s: %s", #call, hipGetErrorString(e_));                                       \
+            return bail(code_);                                                                             \
+        }                                                                                                   \
+    } while (0)
+
+    CREATE_TRY(hipGetDevice(&ctx->device));
+    CREATE_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    const size_t n = ctx->n;
+    CREATE_TRY(dalloc(&ctx->x, n));
+    CREATE_TRY(dalloc(&ctx->x_alt, n));
+    CREATE_TRY(dalloc(&ctx->xs[0], n));
+    CREATE_TRY(dalloc(&ctx->xs[1], n));
+    CREATE_TRY(dalloc(&ctx->v, n));
+    CREATE_TRY(dalloc(&ctx->v_alt, n));
+    CREATE_TRY(dalloc(&ctx->omega, n));
+    CREATE_TRY(dalloc(&ctx->rho, n));
+    CREATE_TRY(dalloc(&ctx->lambda, n));
+    CREATE_TRY(dalloc(&ctx->keys_in, n));
+    CREATE_TRY(dalloc(&ctx->keys_tmp, n));
+    CREATE_TRY(dalloc(&ctx->keys_sorted, n));
+    CREATE_TRY(dalloc(&ctx->vals_a, n));
+    CREATE_TRY(dalloc(&ctx->vals_b, n));
+    CREATE_TRY(dalloc(&ctx->perm, n));
+    CREATE_TRY(dalloc(&ctx->cells, 2 * (size_t)ctx->total_cells));
+    CREATE_TRY(dalloc(&ctx->nb_cnt, n));
+    CREATE_TRY(dalloc(&ctx->nb_list, n * pbf::kCircles * (size_t)pbf::kSeg));
+    CREATE_TRY(dalloc(&ctx->nb_quads, ((n + 63) / 64) * 64 * (size_t)pbf::kQuads));
+    CREATE_TRY(dalloc(&ctx->err, pbf::kErrWords));
+    CREATE_TRY(dalloc(&ctx->sc_d, S));
+    CREATE_TRY(hipMemcpy(ctx->sc_d, ctx->sc.data(), sizeof(pbf::SceneConsts) * S, hipMemcpyHostToDevice));
+
+    for (int k = 0; k < pbf::kNumBlockSizes; ++k) {
+        const std::vector<uint4> t = block_table(ctx, 64u << k);
+        CREATE_TRY(dalloc(&ctx->bt[k].d, t.size()));
+        if (!t.empty()) CREATE_TRY(hipMemcpy(ctx->bt[k].d, t.data(), sizeof(uint4) * t.size(), hipMemcpyHostToDevice));
+        ctx->bt[k].count = (uint32_t)t.size();
+    }
+    {
+        const std::vector<uint4> t = block_table(ctx, 256u * pbf::kPredictItems);
+        CREATE_TRY(dalloc(&ctx->bt_predict.d, t.size()));
+        if (!t.empty())
+            CREATE_TRY(hipMemcpy(ctx->bt_predict.d, t.data(), sizeof(uint4) * t.size(), hipMemcpyHostToDevice));
+        ctx->bt_predict.count = (uint32_t)t.size();
+    }
+
+    // sort geometry: enough 8-bit passes for keys < total_cells
+    uint32_t bits = 0;
+    while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)ctx->total_cells) ++bits;
+    if (bits == 0) bits = 1;
+    ctx->sort.bits = bits;
+    ctx->sort.passes = (bits + pbf::kRadixBits - 1) / pbf::kRadixBits;
+    ctx->sort.tiles = (uint32_t)std::max<size_t>((n + pbf::kSortTile - 1) / pbf::kSortTile, 1);
+    const size_t hist_words = (size_t)pbf::kMaxPasses * pbf::kRadix;
+    const size_t ctr_words = 64;
+    const size_t status_words = (size_t)ctx->sort.passes * ctx->sort.tiles * pbf::kRadix;
+    ctx->sort.ws_words = hist_words + ctr_words + status_words;
+    CREATE_TRY(dalloc(&ctx->sort.ws, ctx->sort.ws_words));
+    ctx->sort.hist = ctx->sort.ws;
+    ctx->sort.ctr = ctx->sort.ws + hist_words;
+    ctx->sort.status = ctx->sort.ws + hist_words + ctr_words;
+
+    CREATE_TRY(hipMemsetAsync(ctx->cells, 0xFF, 8 * (size_t)ctx->total_cells, ctx->stream));
+    CREATE_TRY(hipMemsetAsync(ctx->nb_cnt, 0, 4 * std::max<size_t>(n, 1), ctx->stream));
+    CREATE_TRY(hipMemsetAsync(ctx->v, 0, 16 * std::max<size_t>(n, 1), ctx->stream));
+    CREATE_TRY(hipMemsetAsync(ctx->err, 0, 4 * pbf::kErrWords, ctx->stream));
+    CREATE_TRY(hipStreamSynchronize(ctx->stream));
+#undef CREATE_TRY
+    *out = ctx;
+    return PBF_OK;
+}
+
+// The state of the buffer rotation after a step: what a captured graph
+// leaves behind, applied on the host when the graph is replayed.
+struct Rotation {
+    float4 *x, *x_alt, *v, *v_alt;
+    int xs_cur;
+};
+
+Rotation rotation_of(const pbf_ctx* ctx) { return {ctx->x, ctx->x_alt, ctx->v, ctx->v_alt, ctx->xs_cur}; }
+
+int enqueue_step(pbf_ctx* ctx, uint32_t pause) {
+    ctx->pause_now = pause ? 1u : 0u;
+    // Simulation.cpp:563-567: predict always; radixsort unless paused.  With
+    // the particle order not sorted (no step yet, or new state) the step
+    // sorts anyway.
+    if (pause && ctx->order_sorted) {
+        HIP_TRY(ctx, pbf::launch_predict_only(ctx, 1u));
+    } else {
+        HIP_TRY(ctx, pbf::launch_clear(ctx));
+        HIP_TRY(ctx, pbf::launch_predict_hash(ctx, ctx->cells_hold_keys));
+        HIP_TRY(ctx, pbf::launch_sort(ctx));
+        HIP_TRY(ctx, pbf::launch_gather(ctx));
+        HIP_TRY(ctx, pbf::launch_cells(ctx));
+        ctx->cells_hold_keys = true;
+        ctx->order_sorted = true;
+    }
+    HIP_TRY(ctx, pbf::launch_friends(ctx));
+    const int K = (int)ctx->iters;
+    for (int it = 0; it < K; ++it) {
+        HIP_TRY(ctx, pbf::launch_lambda(ctx, it));
+        HIP_TRY(ctx, pbf::launch_delta(ctx, it, it == K - 1));
+    }
+    HIP_TRY(ctx, pbf::launch_viscosity(ctx));
+    HIP_TRY(ctx, pbf::launch_vorticity(ctx));
+    return PBF_OK;
+}
+
+// Graph key: everything that changes the launch sequence or its arguments.
+uint32_t graph_key(const pbf_ctx* ctx, uint32_t pause) {
+    const bool sorts = !(pause && ctx->order_sorted);
+    return (ctx->x == ctx->x_base ? 1u : 0u) | (ctx->v == ctx->v_base ? 2u : 0u) | (sorts ? 4u : 0u) |
+           (ctx->cells_hold_keys ? 8u : 0u) | (pause ? 16u : 0u) | ((uint32_t)ctx->xs_cur << 5);
+}
+
+int step_graph(pbf_ctx* ctx, uint32_t pause) {
+    const uint32_t key = graph_key(ctx, pause);
+    for (pbf::GraphEntry& g : ctx->graphs) {
+        if (g.key != key) continue;
+        HIP_TRY(ctx, hipGraphLaunch(g.exec, ctx->stream));
+        ctx->x = g.post_x; ctx->x_alt = g.post_x_alt; ctx->v = g.post_v; ctx->v_alt = g.post_v_alt;
+        ctx->xs_cur = g.post_xs_cur;
+        ctx->cells_hold_keys = ctx->cells_hold_keys || g.sorts;
+        ctx->order_sorted = ctx->order_sorted || g.sorts;
+        return PBF_OK;
+    }
+    // capture this variant once
+    const bool hold = ctx->cells_hold_keys, ord = ctx->order_sorted;
+    const Rotation before = rotation_of(ctx);
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step(ctx, pause);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != PBF_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(ctx, PBF_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+    pbf::GraphEntry g;
+    g.key = key;
+    g.sorts = (key & 4u) != 0;
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(ctx, PBF_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    g.post_x = ctx->x; g.post_x_alt = ctx->x_alt; g.post_v = ctx->v; g.post_v_alt = ctx->v_alt;
+    g.post_xs_cur = ctx->xs_cur;
+    ctx->graphs.push_back(g);
+    // the capture recorded the work without running it: rewind and launch
+    ctx->x = before.x; ctx->x_alt = before.x_alt; ctx->v = before.v; ctx->v_alt = before.v_alt;
+    ctx->xs_cur = before.xs_cur;
+    ctx->cells_hold_keys = hold;
+    ctx->order_sorted = ord;
+    return step_graph(ctx, pause);
 }
 
 }  // namespace
@@ -176,7 +434,7 @@ void collect_timings(pbf_ctx* ctx) {
 
 extern "C" {
 
-const char* pbf_version(void) { return "pbf_amd 0.1 (gfx950)"; }
+const char* pbf_version(void) { return "pbf_amd 0.2 (gfx950)"; }
 
 void pbf_default_options(pbf_options* o) {
     if (!o) return;
@@ -192,83 +450,26 @@ void pbf_params_finalize(pbf_params* p) {
 }
 
 int pbf_create(const pbf_params* params, const pbf_options* opt, pbf_ctx** out) {
-    if (!out) return fail(nullptr, PBF_E_INVALID, "out is NULL");
-    *out = nullptr;
-    int rc = validate(params, nullptr);
-    if (rc) return rc;
-    pbf_ctx* ctx = new (std::nothrow) pbf_ctx();
-    if (!ctx) return fail(nullptr, PBF_E_NOMEM, "host allocation failed");
-    ctx->P = *params;
-    if (opt)
-        ctx->opt = *opt;
-    else
-        pbf_default_options(&ctx->opt);
-    ctx->c = make_consts(ctx->P, ctx->opt);
-    ctx->n = params->particleCount;
+    return create_impl(params, opt, 1, out);
+}
 
-    auto bail = [&](int code) {
-        g_create_error = ctx->last_error;
-        free_all(ctx);
-        delete ctx;
-        return code;
-    };
-#define CREATE_TRY(call)                                                                                    \
-    do {                                                                                                    \
-        hipError_t e_ = (call);                                                                             \
-        if (e_ != hipSuccess) {                                                                             \
-            fail(ctx, e_ == hipErrorOutOfMemory ? PBF_E_NOMEM : PBF_E_HIP, "%s: %s", #call,                 \
-                 hipGetErrorString(e_));                                                                    \
-            return bail(e_ == hipErrorOutOfMemory ? PBF_E_NOMEM : PBF_E_HIP);                               \
-        }                                                                                                   \
-    } while (0)
+int pbf_create_batch(const pbf_params* params, const pbf_options* opts, uint32_t scenes, pbf_ctx** out) {
+    return create_impl(params, opts, scenes, out);
+}
 
-    CREATE_TRY(hipGetDevice(&ctx->device));
-    CREATE_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    const size_t n = ctx->n;
-    CREATE_TRY(dalloc(&ctx->x, n));
-    CREATE_TRY(dalloc(&ctx->x_alt, n));
-    CREATE_TRY(dalloc(&ctx->xs[0], n));
-    CREATE_TRY(dalloc(&ctx->xs[1], n));
-    CREATE_TRY(dalloc(&ctx->v, n));
-    CREATE_TRY(dalloc(&ctx->v_alt, n));
-    CREATE_TRY(dalloc(&ctx->omega, n));
-    CREATE_TRY(dalloc(&ctx->rho, n));
-    CREATE_TRY(dalloc(&ctx->lambda, n));
-    CREATE_TRY(dalloc(&ctx->keys_in, n));
-    CREATE_TRY(dalloc(&ctx->keys_tmp, n));
-    CREATE_TRY(dalloc(&ctx->keys_sorted, n));
-    CREATE_TRY(dalloc(&ctx->vals_a, n));
-    CREATE_TRY(dalloc(&ctx->vals_b, n));
-    CREATE_TRY(dalloc(&ctx->perm, n));
-    CREATE_TRY(dalloc(&ctx->cells, 2 * (size_t)params->gridBufSize));
-    CREATE_TRY(dalloc(&ctx->nb_cnt, n));
-    CREATE_TRY(dalloc(&ctx->nb_idx, n * pbf::kCircles * pbf::kPerCircle));
-    CREATE_TRY(dalloc(&ctx->err, pbf::kErrWords));
+int pbf_scene_offsets(pbf_ctx* ctx, uint32_t* offsets) {
+    if (!ctx || !offsets) return PBF_E_INVALID;
+    for (uint32_t s = 0; s < ctx->S; ++s) offsets[s] = ctx->sc[s].first;
+    offsets[ctx->S] = ctx->n;
+    return PBF_OK;
+}
 
-    // sort geometry: enough 8-bit passes for keys < gridBufSize
-    uint32_t bits = 0;
-    while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)params->gridBufSize) ++bits;
-    if (bits == 0) bits = 1;
-    ctx->sort.bits = bits;
-    ctx->sort.passes = (bits + pbf::kRadixBits - 1) / pbf::kRadixBits;
-    ctx->sort.tiles = (uint32_t)((n + pbf::kSortTile - 1) / pbf::kSortTile);
-    if (ctx->sort.tiles == 0) ctx->sort.tiles = 1;
-    const size_t hist_words = (size_t)pbf::kMaxPasses * pbf::kRadix;
-    const size_t ctr_words = 64;
-    const size_t status_words = (size_t)ctx->sort.passes * ctx->sort.tiles * pbf::kRadix;
-    ctx->sort.ws_words = hist_words + ctr_words + status_words;
-    CREATE_TRY(dalloc(&ctx->sort.ws, ctx->sort.ws_words));
-    ctx->sort.hist = ctx->sort.ws;
-    ctx->sort.ctr = ctx->sort.ws + hist_words;
-    ctx->sort.status = ctx->sort.ws + hist_words + ctr_words;
+int pbf_scene_count(pbf_ctx* ctx) { return ctx ? (int)ctx->S : PBF_E_INVALID; }
 
-    CREATE_TRY(hipMemsetAsync(ctx->cells, 0xFF, 8 * (size_t)params->gridBufSize, ctx->stream));
-    CREATE_TRY(hipMemsetAsync(ctx->nb_cnt, 0, 4 * std::max<size_t>(n, 1), ctx->stream));
-    CREATE_TRY(hipMemsetAsync(ctx->v, 0, 16 * std::max<size_t>(n, 1), ctx->stream));
-    CREATE_TRY(hipMemsetAsync(ctx->err, 0, 4 * pbf::kErrWords, ctx->stream));
-    CREATE_TRY(hipStreamSynchronize(ctx->stream));
-#undef CREATE_TRY
-    *out = ctx;
+int pbf_friend_circle_thresholds(float h, float* t4) {
+    if (!t4 || !(h > 0.0f)) return PBF_E_INVALID;
+    const float h2 = h * h, min_r = 0.3f * h, span = h - min_r;
+    circle_thresholds(h2, min_r, span, t4);
     return PBF_OK;
 }
 
@@ -294,7 +495,7 @@ int pbf_set_state(pbf_ctx* ctx, const float* x4, const float* v4, uint32_t n) {
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
     ctx->have_state = true;
-    ctx->have_sorted = false;
+    ctx->order_sorted = false;   // cells still hold the old keys until reset
     return PBF_OK;
 }
 
@@ -302,28 +503,12 @@ int pbf_step(pbf_ctx* ctx, uint32_t pause) {
     if (!ctx) return PBF_E_INVALID;
     if (!ctx->have_state) return fail(ctx, PBF_E_STATE, "pbf_step before pbf_set_state");
     if (ctx->n == 0) return PBF_OK;
-    ctx->pause_now = pause ? 1u : 0u;
-    // Simulation.cpp:563-567: predict always; radixsort unless paused.  With
-    // no sort yet there are no sorted keys, so the first step always sorts.
-    if (pause && ctx->have_sorted) {
-        HIP_TRY(ctx, pbf::launch_predict_only(ctx, 1u));
-    } else {
-        HIP_TRY(ctx, pbf::launch_clear(ctx));
-        HIP_TRY(ctx, pbf::launch_predict_hash(ctx, ctx->have_sorted));
-        HIP_TRY(ctx, pbf::launch_sort(ctx));
-        HIP_TRY(ctx, pbf::launch_gather(ctx));
-        HIP_TRY(ctx, pbf::launch_cells(ctx));
-        ctx->have_sorted = true;
+    if (ctx->x_base == nullptr) {
+        ctx->x_base = ctx->x;
+        ctx->v_base = ctx->v;
     }
-    HIP_TRY(ctx, pbf::launch_friends(ctx));
-    const int K = (int)ctx->P.simIterations;
-    for (int it = 0; it < K; ++it) {
-        HIP_TRY(ctx, pbf::launch_lambda(ctx, it));
-        HIP_TRY(ctx, pbf::launch_delta(ctx, it, it == K - 1));
-    }
-    HIP_TRY(ctx, pbf::launch_viscosity(ctx));
-    HIP_TRY(ctx, pbf::launch_vorticity(ctx));
-    return PBF_OK;
+    if (ctx->use_graph && !ctx->profiling) return step_graph(ctx, pause);
+    return enqueue_step(ctx, pause);
 }
 
 int pbf_run(pbf_ctx* ctx, uint32_t steps, uint32_t pause) {
@@ -331,6 +516,14 @@ int pbf_run(pbf_ctx* ctx, uint32_t steps, uint32_t pause) {
         const int rc = pbf_step(ctx, pause);
         if (rc) return rc;
     }
+    return PBF_OK;
+}
+
+int pbf_set_graph(pbf_ctx* ctx, int enable) {
+    if (!ctx) return PBF_E_INVALID;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (!enable) drop_graphs(ctx);
+    ctx->use_graph = enable != 0;
     return PBF_OK;
 }
 
@@ -360,22 +553,22 @@ int pbf_get_structure(pbf_ctx* ctx, uint32_t* keys_sorted, uint32_t* perm, uint3
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (keys_sorted && n) HIP_TRY(ctx, hipMemcpy(keys_sorted, ctx->keys_sorted, 4 * n, hipMemcpyDeviceToHost));
     if (perm && n) HIP_TRY(ctx, hipMemcpy(perm, ctx->perm, 4 * n, hipMemcpyDeviceToHost));
-    if (cells)
-        HIP_TRY(ctx, hipMemcpy(cells, ctx->cells, 8 * (size_t)ctx->P.gridBufSize, hipMemcpyDeviceToHost));
+    if (cells) HIP_TRY(ctx, hipMemcpy(cells, ctx->cells, 8 * (size_t)ctx->total_cells, hipMemcpyDeviceToHost));
     if (friends && n) {
         // counts[5][n] unpacked from the 6-bit fields, then indices[5][50][n]
+        // regrouped from the particle-major circle segments (zero past each count)
         std::vector<uint32_t> packed(n);
         HIP_TRY(ctx, hipMemcpy(packed.data(), ctx->nb_cnt, 4 * n, hipMemcpyDeviceToHost));
-        for (int q = 0; q < pbf::kCircles; ++q)
-            for (size_t i = 0; i < n; ++i) friends[(size_t)q * n + i] = (packed[i] >> (6 * q)) & 63u;
+        std::vector<uint32_t> segs(n * pbf::kCircles * (size_t)pbf::kSeg);
+        HIP_TRY(ctx, hipMemcpy(segs.data(), ctx->nb_list, 4 * segs.size(), hipMemcpyDeviceToHost));
         uint32_t* idx = friends + (size_t)pbf::kCircles * n;
-        HIP_TRY(ctx, hipMemcpy(idx, ctx->nb_idx, 4 * n * pbf::kCircles * pbf::kPerCircle, hipMemcpyDeviceToHost));
-        // slots past each circle's count hold stale data; zero them
-        for (int q = 0; q < pbf::kCircles; ++q)
-            for (int f = 0; f < pbf::kPerCircle; ++f) {
-                uint32_t* row = idx + ((size_t)q * pbf::kPerCircle + f) * n;
-                for (size_t i = 0; i < n; ++i)
-                    if ((uint32_t)f >= friends[(size_t)q * n + i]) row[i] = 0;
+        memset(idx, 0, 4 * n * pbf::kCircles * pbf::kPerCircle);
+        for (size_t i = 0; i < n; ++i)
+            for (int q = 0; q < pbf::kCircles; ++q) {
+                const uint32_t cnt = (packed[i] >> (6 * q)) & 63u;
+                friends[(size_t)q * n + i] = cnt;
+                const uint32_t* seg = segs.data() + (i * pbf::kCircles + q) * pbf::kSeg;
+                for (uint32_t f = 0; f < cnt; ++f) idx[((size_t)q * pbf::kPerCircle + f) * n + i] = seg[f];
             }
     }
     return PBF_OK;
@@ -411,6 +604,14 @@ int pbf_timings(pbf_ctx* ctx, pbf_timing* out, int cap, int* count) {
     return PBF_OK;
 }
 
+int pbf_reset_timings(pbf_ctx* ctx) {
+    if (!ctx) return PBF_E_INVALID;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    pbf::collect_timings(ctx);
+    ctx->trackers.clear();
+    return PBF_OK;
+}
+
 int pbf_error_flags(pbf_ctx* ctx, uint32_t* flags, uint64_t* dropped, int reset) {
     if (!ctx) return PBF_E_INVALID;
     uint32_t w[pbf::kErrWords];
@@ -427,6 +628,135 @@ int pbf_error_flags(pbf_ctx* ctx, uint32_t* flags, uint64_t* dropped, int reset)
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
     return PBF_OK;
+}
+
+int pbf_stats(pbf_ctx* ctx, pbf_stats_t* out) {
+    if (!ctx || !out) return PBF_E_INVALID;
+    uint32_t w[pbf::kErrWords];
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(w, ctx->err, sizeof w, hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof *out);
+    out->unstaged_tiles = w[4];
+    out->friends_staged = ctx->friends_staged ? 1u : 0u;
+    out->sort_passes = ctx->sort.passes;
+    out->sort_tiles = ctx->sort.tiles;
+    out->total_cells = ctx->total_cells;
+    out->particles = ctx->n;
+    out->scenes = ctx->S;
+    for (int f = 0; f < pbf::F_COUNT; ++f) out->workgroup_size[f] = 1u << ctx->bs_log2[f];
+    return PBF_OK;
+}
+
+int pbf_set_friends_mode(pbf_ctx* ctx, int staged) {
+    if (!ctx) return PBF_E_INVALID;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->friends_staged = staged != 0;
+    return PBF_OK;
+}
+
+int pbf_set_workgroup_sizes(pbf_ctx* ctx, const uint32_t* sizes) {
+    if (!ctx || !sizes) return PBF_E_INVALID;
+    int lg[pbf::F_COUNT];
+    for (int f = 0; f < pbf::F_COUNT; ++f) {
+        int l = 6;
+        while (l <= 9 && (1u << l) != sizes[f]) ++l;
+        if (l > 9) return fail(ctx, PBF_E_INVALID, "workgroup size %u is not one of 64, 128, 256, 512", sizes[f]);
+        lg[f] = l;
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    for (int f = 0; f < pbf::F_COUNT; ++f) ctx->bs_log2[f] = lg[f];
+    return PBF_OK;
+}
+
+// Times every workgroup size for every family on the current state and keeps
+// the fastest per family.  The particle state is restored afterwards; the
+// next step re-sorts (the order is no longer known to be sorted).
+int pbf_autotune(pbf_ctx* ctx, uint32_t steps, uint32_t* chosen) {
+    if (!ctx) return PBF_E_INVALID;
+    if (!ctx->have_state) return fail(ctx, PBF_E_STATE, "pbf_autotune before pbf_set_state");
+    if (ctx->n == 0) return PBF_OK;
+    if (steps == 0) steps = 1;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t bytes = 16 * (size_t)ctx->n;
+    float4 *sx = nullptr, *sv = nullptr;
+    HIP_TRY(ctx, hipMalloc((void**)&sx, bytes));
+    if (hipMalloc((void**)&sv, bytes) != hipSuccess) {
+        (void)hipFree(sx);
+        return fail(ctx, PBF_E_NOMEM, "pbf_autotune: no memory for the state copy");
+    }
+    int rc = PBF_OK;
+    const bool prev_graph = ctx->use_graph, prev_prof = ctx->profiling;
+    ctx->use_graph = false;
+    pbf::collect_timings(ctx);
+    const std::vector<pbf::Tracker> saved_trackers = ctx->trackers;
+    double best[pbf::F_COUNT];
+    int best_l[pbf::F_COUNT];
+    for (int f = 0; f < pbf::F_COUNT; ++f) {
+        best[f] = 1e300;
+        best_l[f] = ctx->bs_log2[f];
+    }
+    auto restore = [&]() -> int {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->x, sx, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->v, sv, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->order_sorted = false;
+        return PBF_OK;
+    };
+    static const char* kFamilyTrackers[pbf::F_COUNT] = {"buildFriendsList", "computeScaling", "computeDelta",
+                                                        "applyViscosity", "applyVorticity"};
+    if (hipMemcpyAsync(sx, ctx->x, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(sv, ctx->v, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+        rc = fail(ctx, PBF_E_HIP, "pbf_autotune: state copy failed");
+    }
+    double best_mode_ms[2] = {1e300, 1e300};
+    int best_mode_l[2] = {8, 8};
+    for (int trial = 0; trial < 8 && rc == PBF_OK; ++trial) {
+        const int l = 6 + (trial & 3);
+        const bool staged_trial = trial >= 4;
+        ctx->friends_staged = staged_trial;
+        for (int f = 0; f < pbf::F_COUNT; ++f) ctx->bs_log2[f] = l;
+        if ((rc = restore()) != PBF_OK) break;
+        ctx->profiling = false;
+        if ((rc = enqueue_step(ctx, 0)) != PBF_OK) break;   // warm-up
+        ctx->trackers.clear();
+        ctx->profiling = true;
+        for (uint32_t s = 0; s < steps && rc == PBF_OK; ++s) rc = enqueue_step(ctx, 0);
+        if (rc != PBF_OK) break;
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) { rc = fail(ctx, PBF_E_HIP, "pbf_autotune: sync"); break; }
+        pbf::collect_timings(ctx);
+        ctx->profiling = false;
+        for (int f = 0; f < pbf::F_COUNT; ++f) {
+            double ms = 0.0;
+            const size_t len = strlen(kFamilyTrackers[f]);
+            for (const pbf::Tracker& t : ctx->trackers)
+                if (t.name.compare(0, len, kFamilyTrackers[f]) == 0) ms += t.t.sum_ms;
+            if (f == pbf::F_FRIENDS) {
+                if (ms < best_mode_ms[staged_trial]) {
+                    best_mode_ms[staged_trial] = ms;
+                    best_mode_l[staged_trial] = l;
+                }
+            } else if (!staged_trial && ms < best[f]) {   // the other families run in every trial
+                best[f] = ms;
+                best_l[f] = l;
+            }
+        }
+    }
+    ctx->friends_staged = best_mode_ms[1] < best_mode_ms[0];
+    best_l[pbf::F_FRIENDS] = best_mode_l[ctx->friends_staged ? 1 : 0];
+    for (int f = 0; f < pbf::F_COUNT; ++f) {
+        ctx->bs_log2[f] = best_l[f];
+        if (chosen) chosen[f] = 1u << best_l[f];
+    }
+    if (rc == PBF_OK) rc = restore();
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(sx);
+    (void)hipFree(sv);
+    ctx->trackers = saved_trackers;
+    ctx->profiling = prev_prof;
+    drop_graphs(ctx);
+    ctx->use_graph = prev_graph;
+    return rc;
 }
 
 }  // extern "C"

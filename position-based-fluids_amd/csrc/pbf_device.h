@@ -14,20 +14,22 @@ constexpr int kCircles = 5;          // Params.friendsCircles (src/ParamUtils.cp
 constexpr int kPerCircle = 50;       // Params.particlesPerCircle (src/ParamUtils.cpp:99)
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // END_OF_CELL_LIST (src/Simulation.cpp:170)
 
-// Per-step constants, passed by value to every kernel.
-struct StepConsts {
+// Constants of one scene.  Kernels read them through a workgroup-uniform
+// index (the block table), so they land in scalar registers.
+struct SceneConsts {
     float h, h2, min_r, span;       // span = h - 0.3h (build_friends_list.cl:32,75)
-    float dt;
-    float g[3];
-    float rest_density, epsilon;
-    float poly6, spiky;             // the 6-significant-digit factors
-    float st_k, poly6_q;            // s_corr constants (compute_delta.cl:81-82,152)
-    float viscosity, vorticity;
-    float lo[3], hi[3];
-    uint32_t walls;
-    uint32_t gbs, gbs_mask;         // gbs_mask != 0 when gbs is a power of two
-    uint32_t n;
+    float dt, g0, g1, g2;
+    float rest_density, epsilon, poly6, spiky;  // poly6/spiky: the 6-digit factors
+    float st_k, poly6_q, viscosity, vorticity;  // s_corr constants (compute_delta.cl:81-82,152)
+    float lo0, lo1, lo2, hi0;
+    float hi1, hi2;
+    uint32_t walls, gbs;
+    uint32_t gbs_mask;              // != 0 when gbs is a power of two
+    uint32_t cell_base;             // first global cell id of this scene
+    uint32_t first, count;          // particle range of this scene
+    float circ_t[4];                // r^2 at which the friend circle reaches 1..4
 };
+static_assert(sizeof(SceneConsts) == 128, "SceneConsts is 32 words");
 
 // utilities.cl:69-77 on the 32-bit two's-complement pattern.
 __device__ __forceinline__ uint32_t expand_bits(int32_t xi) {
@@ -39,11 +41,12 @@ __device__ __forceinline__ uint32_t expand_bits(int32_t xi) {
     return x;
 }
 
-// utilities.cl:79-87.  The Morton number is below 2^30, so the signed modulo
-// of the reference equals an unsigned one, and a mask when gbs = 2^k.
-__device__ __forceinline__ uint32_t cell_hash(int32_t cx, int32_t cy, int32_t cz, const StepConsts& c) {
+// utilities.cl:79-87, offset by the scene's first cell.  The Morton number is
+// below 2^30, so the reference's signed modulo equals an unsigned one, and a
+// mask when gbs = 2^k.
+__device__ __forceinline__ uint32_t cell_hash(int32_t cx, int32_t cy, int32_t cz, const SceneConsts& c) {
     const uint32_t m = expand_bits(cx) | (expand_bits(cy) << 1) | (expand_bits(cz) << 2);
-    return c.gbs_mask ? (m & c.gbs_mask) : (m % c.gbs);
+    return c.cell_base + (c.gbs_mask ? (m & c.gbs_mask) : (m % c.gbs));
 }
 
 // convert_int of x/h (radixsort.cl:12): truncation toward zero; NaN -> 0 and
@@ -74,31 +77,58 @@ __device__ __forceinline__ float noise_sin(float x) {
     return (((int32_t)k) & 1) ? -p : p;
 }
 
+__device__ __forceinline__ int circle_count(uint32_t packed, int q) { return (packed >> (6 * q)) & 63; }
+
 // Number of friend circles a sweep processes.  The reference skips circle q
 // when (float)processed/total > threshold (compute_scaling.cl:50, 0.6;
 // compute_delta.cl:100, 0.5); once one circle is skipped `processed` stops
 // growing, so the processed circles form a prefix.  For counts <= 250 the
-// fp32 test equals the exact rational one used here.
+// fp32 test equals the exact rational one used here.  Returns the prefix
+// length; *m receives the number of friends in it.
 template <int NUM, int DEN>  // threshold = NUM/DEN
-__device__ __forceinline__ int circles_processed(uint32_t packed) {
-    int c[kCircles];
+__device__ __forceinline__ int circles_processed(uint32_t packed, int* m) {
     int total = 0;
 #pragma unroll
-    for (int q = 0; q < kCircles; ++q) { c[q] = (packed >> (6 * q)) & 63; total += c[q]; }
+    for (int q = 0; q < kCircles; ++q) total += circle_count(packed, q);
     int done = 0, L = 0;
 #pragma unroll
     for (int q = 0; q < kCircles; ++q) {
-        if (L == q && !(DEN * done > NUM * total)) { done += c[q]; L = q + 1; }
+        if (L == q && !(DEN * done > NUM * total)) { done += circle_count(packed, q); L = q + 1; }
     }
+    *m = done;
     return L;
 }
 
-__device__ __forceinline__ int circle_count(uint32_t packed, int q) { return (packed >> (6 * q)) & 63; }
+// Friends lists, particle-major: particle i owns five 16-byte aligned
+// segments of kSeg slots, segment c holding its circle-c friends in the scan
+// order of the reference.  This is the reference's layout (counts[5][n],
+// indices[5][50][n], build_friends_list.cl:1-21) with the particle index
+// moved outermost: a thread appends to its own segments as it finds friends
+// (each cache line is filled by one thread) and a sweep reads four friends
+// per 16-byte load.
+constexpr int kSeg = 52;
+__device__ __forceinline__ size_t friend_seg(uint32_t i, int circle) {
+    return ((size_t)i * kCircles + (size_t)circle) * kSeg;
+}
 
-// Friend index slot in the reference layout (build_friends_list.cl:18-21):
-// indices[circle][f][i] after the counts block.
-__device__ __forceinline__ size_t friend_slot(int q, int f, uint32_t i, uint32_t n) {
-    return ((size_t)q * kPerCircle + (size_t)f) * n + i;
+// Sweep lists: the friends a sweep processes (the 0.6-threshold prefix, in
+// the reference's order) regrouped per block of 64 consecutive particles in
+// quads: quad q of the block holds friends 4q..4q+3 of each of its 64
+// particles as one uint4 per particle, so a wave's load of one quad is a
+// contiguous 1 KB and gives every lane four friends.
+constexpr int kQuads = (kCircles * kPerCircle + 3) / 4;
+__device__ __forceinline__ size_t sweep_quad(uint32_t i, int q) {
+    return ((size_t)(i >> 6) * kQuads + (size_t)q) * 64 + (i & 63u);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip; this
+// maps dispatch index b of nb workgroups to a tile so that each XCD takes one
+// contiguous run of tiles.  Neighbouring particles of a tile then sit in the
+// same XCD's L2 as the tile.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
 }
 
 // Box wall planes: BouncePointQuad (compute_delta.cl:1-42) for an infinite
@@ -121,17 +151,17 @@ __device__ __forceinline__ float wall_hi(float f, float prev, float hi, float ed
     return pa;
 }
 
-__device__ __forceinline__ void apply_walls(float& fx, float& fy, float& fz, const float4& prev, const StepConsts& c) {
+__device__ __forceinline__ void apply_walls(float& fx, float& fy, float& fz, const float4& prev, const SceneConsts& c) {
     const float nx = noise_sin(fx * 5.0f / c.h);
     const float ny = noise_sin(fy * 5.0f / c.h);
     const float nz = noise_sin(fz * 5.0f / c.h);
     const float edge = (3.0f + nx + ny + nz) * c.h * 0.03f;
-    fx = wall_lo(fx, prev.x, c.lo[0], edge);
-    fx = wall_hi(fx, prev.x, c.hi[0], edge);
-    fy = wall_lo(fy, prev.y, c.lo[1], edge);
-    fy = wall_hi(fy, prev.y, c.hi[1], edge);
-    fz = wall_lo(fz, prev.z, c.lo[2], edge);
-    fz = wall_hi(fz, prev.z, c.hi[2], edge);
+    fx = wall_lo(fx, prev.x, c.lo0, edge);
+    fx = wall_hi(fx, prev.x, c.hi0, edge);
+    fy = wall_lo(fy, prev.y, c.lo1, edge);
+    fy = wall_hi(fy, prev.y, c.hi1, edge);
+    fz = wall_lo(fz, prev.z, c.lo2, edge);
+    fz = wall_hi(fz, prev.z, c.hi2, edge);
 }
 
 }  // namespace pbf

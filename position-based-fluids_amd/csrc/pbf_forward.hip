@@ -6,32 +6,36 @@
 //   (onesweep passes, pbf_sort.hip)
 //   k_gather         sortParticles; recomputes x* from the gathered x, v
 //   k_cells          updateCells
-//   k_friends        buildFriendsList
+//   k_friends        buildFriendsList, 27-cell neighbourhood staged in LDS
 //   K x { k_lambda   computeScaling + packData(lambda)
 //         k_delta    computeDelta (+ walls) + updatePredicted; on the last
 //                    iteration also packData(density) + updateVelocities }
 //   k_viscosity      applyViscosity (Jacobi: reads a snapshot of v)
 //   k_vorticity      applyVorticity
-// Particles live in cell-sorted order as float4 arrays; the friends list
-// keeps the reference layout (counts packed 5 x 6 bits per particle,
-// indices[5][50][n]) so a wave reads one friend slot of 64 consecutive
-// particles as one contiguous 256-byte segment.
+//
+// Every per-particle kernel runs over a block table (one {scene, first, end}
+// per workgroup), so one launch covers all scenes of a batch and the scene
+// constants of a workgroup are uniform (scalar loads).  Particles live in
+// cell-sorted order as float4 arrays; the friends list keeps the reference
+// layout (counts packed 5 x 6 bits per particle, indices[5][50][n]) so a wave
+// reads one friend slot of 64 consecutive particles as one 256-byte segment.
+// The sweeps walk the friends of a particle four at a time: four index loads,
+// then four gathers, then the four updates in the reference's order.
 #include "pbf_internal.h"
 
 namespace pbf {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kPredictItems = 4;
+constexpr int kPredictThreads = 256;
 
-__device__ __forceinline__ float4 predicted_from(const float4 x, const float4 v, const StepConsts& c, uint32_t pause) {
+__device__ __forceinline__ float4 predicted_from(const float4 x, const float4 v, const SceneConsts& c, uint32_t pause) {
     // predict_positions.cl:12-17
     float4 vg = v;
     if (pause == 0u) {
-        vg.x = v.x + c.dt * c.g[0];
-        vg.y = v.y + c.dt * c.g[1];
-        vg.z = v.z + c.dt * c.g[2];
+        vg.x = v.x + c.dt * c.g0;
+        vg.y = v.y + c.dt * c.g1;
+        vg.z = v.z + c.dt * c.g2;
     }
     float4 o;
     o.x = x.x + c.dt * vg.x;
@@ -41,40 +45,41 @@ __device__ __forceinline__ float4 predicted_from(const float4 x, const float4 v,
     return o;
 }
 
-__device__ __forceinline__ uint32_t key_of(const float4 p, const StepConsts& c) {
+__device__ __forceinline__ uint32_t key_of(const float4 p, const SceneConsts& c) {
     // radixsort.cl:11-13
     return cell_hash(cell_coord(p.x / c.h), cell_coord(p.y / c.h), cell_coord(p.z / c.h), c);
 }
 
-__global__ __launch_bounds__(kThreads) void k_predict_hash(const float4* __restrict__ x, const float4* __restrict__ v,
-                                                          uint32_t* __restrict__ keys,
-                                                          const uint32_t* __restrict__ old_keys,
-                                                          uint2* __restrict__ cells, uint32_t reset_old,
-                                                          uint32_t* __restrict__ hist, uint32_t passes,
-                                                          uint32_t pause, StepConsts c, uint32_t* err) {
+__global__ __launch_bounds__(kPredictThreads) void k_predict_hash(
+    const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ x,
+    const float4* __restrict__ v, uint32_t* __restrict__ keys, const uint32_t* __restrict__ old_keys,
+    uint2* __restrict__ cells, uint32_t reset_old, uint32_t total_cells, uint32_t* __restrict__ hist,
+    uint32_t passes, uint32_t pause, uint32_t* err) {
     __shared__ uint32_t s_hist[kMaxPasses * kRadix];
-    for (int q = threadIdx.x; q < (int)passes * kRadix; q += kThreads) s_hist[q] = 0;
+    const uint4 be = blocks[blockIdx.x];
+    const SceneConsts& c = scenes[be.x];
+    for (int q = threadIdx.x; q < (int)passes * kRadix; q += kPredictThreads) s_hist[q] = 0;
     __syncthreads();
     bool bad = false;
-    const uint32_t base = blockIdx.x * (uint32_t)(kThreads * kPredictItems);
 #pragma unroll
     for (int k = 0; k < kPredictItems; ++k) {
-        const uint32_t i = base + (uint32_t)(k * kThreads) + threadIdx.x;
-        if (i < c.n) {
+        const uint32_t i = be.y + (uint32_t)(k * kPredictThreads) + threadIdx.x;
+        if (i < be.z) {
             const float4 p = predicted_from(x[i], v[i], c, pause);
             const uint32_t key = key_of(p, c);
             keys[i] = key;
-            for (uint32_t q = 0; q < passes; ++q) atomicAdd(&s_hist[q * kRadix + ((key >> (kRadixBits * q)) & (kRadix - 1))], 1u);
+            for (uint32_t q = 0; q < passes; ++q)
+                atomicAdd(&s_hist[q * kRadix + ((key >> (kRadixBits * q)) & (kRadix - 1))], 1u);
             if (reset_old) {
                 // reset_grid.cl: empty the cells of the previous step
                 const uint32_t ok = old_keys[i];
-                if ((i == 0 || old_keys[i - 1] != ok) && ok < c.gbs) cells[ok] = make_uint2(kEmpty, kEmpty);
+                if ((i == 0 || old_keys[i - 1] != ok) && ok < total_cells) cells[ok] = make_uint2(kEmpty, kEmpty);
             }
             bad |= !(fabsf(p.x) <= 3.4e38f && fabsf(p.y) <= 3.4e38f && fabsf(p.z) <= 3.4e38f);
         }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < (int)passes * kRadix; q += kThreads) {
+    for (int q = threadIdx.x; q < (int)passes * kRadix; q += kPredictThreads) {
         const uint32_t h = s_hist[q];
         if (h) atomicAdd(&hist[q], h);
     }
@@ -82,24 +87,32 @@ __global__ __launch_bounds__(kThreads) void k_predict_hash(const float4* __restr
 }
 
 // Paused step with a valid sort: predictPositions only (no re-sort).
-__global__ __launch_bounds__(kThreads) void k_predict_only(const float4* __restrict__ x, const float4* __restrict__ v,
-                                                          float4* __restrict__ xs, uint32_t pause, StepConsts c) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= c.n) return;
+__global__ __launch_bounds__(256) void k_predict_only(const SceneConsts* __restrict__ scenes,
+                                                     const uint4* __restrict__ blocks,
+                                                     const float4* __restrict__ x, const float4* __restrict__ v,
+                                                     float4* __restrict__ xs, uint32_t pause) {
+    const uint4 be = blocks[blockIdx.x];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t i = be.y + threadIdx.x;
+    if (i >= be.z) return;
     xs[i] = predicted_from(x[i], v[i], c, pause);
 }
 
 // radixsort.cl:22-37.  x* is recomputed from the gathered x and v with the
 // same operations as k_predict_hash, so it is bitwise the value that was
 // hashed; v itself is not permuted (it is rewritten before it is read).
-__global__ __launch_bounds__(kThreads) void k_gather(const uint32_t* __restrict__ perm, const float4* __restrict__ x,
-                                                    const float4* __restrict__ v, float4* __restrict__ x_out,
-                                                    float4* __restrict__ xs_out, uint32_t pause, StepConsts c,
-                                                    uint32_t* err) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= c.n) return;
+__global__ __launch_bounds__(256) void k_gather(const SceneConsts* __restrict__ scenes,
+                                               const uint4* __restrict__ blocks,
+                                               const uint32_t* __restrict__ perm, const float4* __restrict__ x,
+                                               const float4* __restrict__ v, float4* __restrict__ x_out,
+                                               float4* __restrict__ xs_out, uint32_t pause, uint32_t n,
+                                               uint32_t* err) {
+    const uint4 be = blocks[blockIdx.x];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t i = be.y + threadIdx.x;
+    if (i >= be.z) return;
     uint32_t p = perm[i];
-    if (p >= c.n) {  // only after a failed sort (flagged there)
+    if (p >= n) {  // only after a failed sort (flagged there)
         atomicOr(err, (uint32_t)PBF_FLAG_INTERNAL);
         p = i;
     }
@@ -108,13 +121,14 @@ __global__ __launch_bounds__(kThreads) void k_gather(const uint32_t* __restrict_
     xs_out[i] = predicted_from(xp, v[p], c, pause);
 }
 
-// update_cells.cl:1-26 (the last particle always closes its cell).
-__global__ __launch_bounds__(kThreads) void k_cells(const uint32_t* __restrict__ ks, uint32_t* __restrict__ cells,
-                                                   uint32_t n, uint32_t gbs, uint32_t* err) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+// update_cells.cl:1-26 over the sorted global keys (the last particle always
+// closes its cell).
+__global__ __launch_bounds__(256) void k_cells(const uint32_t* __restrict__ ks, uint32_t* __restrict__ cells,
+                                              uint32_t n, uint32_t total_cells, uint32_t* err) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = ks[i];
-    if (k >= gbs || (i > 0 && ks[i - 1] >= gbs)) {  // only after a failed sort
+    if (k >= total_cells || (i > 0 && ks[i - 1] >= total_cells)) {  // only after a failed sort
         atomicOr(err, (uint32_t)PBF_FLAG_INTERNAL);
         return;
     }
@@ -130,77 +144,332 @@ __global__ __launch_bounds__(kThreads) void k_cells(const uint32_t* __restrict__
     if (i == n - 1) cells[2 * (size_t)k + 1] = i;
 }
 
-// build_friends_list.cl:23-103.  27 cells in x, y, z order, ascending j;
-// circle = min(int(a^3 * 5), 4), a = max(0, (r - 0.3h) / 0.7h); at most 50
-// friends per circle, later ones dropped (and counted).
-__global__ __launch_bounds__(kThreads) void k_friends(const float4* __restrict__ xs, const uint2* __restrict__ cells,
-                                                     uint32_t* __restrict__ nb_cnt, uint32_t* __restrict__ nb_idx,
-                                                     StepConsts c, uint32_t* err) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= c.n) return;
+// ---------------------------------------------------------------------------
+// buildFriendsList (build_friends_list.cl:23-103).  27 cells in x, y, z order,
+// ascending j; circle = min(int(a^3 * 5), 4), a = max(0, (r - 0.3h) / 0.7h);
+// at most 50 friends per circle, later ones dropped (and counted).  A thread
+// appends each friend to its particle's circle segment as it finds it.
+//
+// Two variants give the same lists.  k_friends_direct reads candidates from
+// global memory, four at a time; the neighbouring tiles are in the same
+// XCD's L2.  k_friends stages its workgroup's neighbourhood in LDS: the BS
+// consecutive sorted particles of a tile span a small cell box (Morton order
+// keeps a run of particles compact); the box grown by one cell is staged
+// (every box cell's hash bucket, start and count, and the bucket's particles
+// in bucket order), and each thread scans its 27 cells from LDS.  Hash-folded
+// cells stage their bucket once per alias, which reproduces the reference's
+// repeated visits.  A tile whose box or particles exceed the LDS budget scans
+// global memory.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Circle of a candidate in range, or -1 when r^2 >= h^2.  The reference's
+// circle, min(int(a^3 * 5), 4) with a = max(0, (sqrt(r^2) - 0.3h) / 0.7h)
+// (build_friends_list.cl:75-76), is a non-decreasing step function of the
+// binary32 r^2 (every operation in it is monotone), so it equals the number
+// of thresholds circ_t[k] <= r^2; the thresholds are found on the host by
+// evaluating the same expression.
+__device__ __forceinline__ int friend_circle(const float4& p, const float4& q, const SceneConsts& c) {
+    const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 >= c.h2) return -1;
+    return (int)(r2 >= c.circ_t[0]) + (int)(r2 >= c.circ_t[1]) + (int)(r2 >= c.circ_t[2]) + (int)(r2 >= c.circ_t[3]);
+}
+
+// Appends friend j of circle `circ` to particle i's list under the 50 cap.
+struct FriendWriter {
+    uint32_t packed = 0, dropped = 0;
+    __device__ __forceinline__ void take(uint32_t i, uint32_t j, int circ, uint32_t* __restrict__ nb_list) {
+        if (circ < 0 || j == i) return;
+        const uint32_t f = (packed >> (6 * circ)) & 63u;
+        if (f >= (uint32_t)kPerCircle) {
+            ++dropped;
+            return;
+        }
+        nb_list[friend_seg(i, circ) + f] = j;
+        packed += 1u << (6 * circ);
+    }
+    __device__ __forceinline__ void finish(uint32_t i, uint32_t* __restrict__ nb_cnt, uint32_t* err) {
+        nb_cnt[i] = packed;
+        if (dropped) {
+            atomicOr(err, (uint32_t)PBF_FLAG_FRIENDS_OVERFLOW);
+            atomicAdd((unsigned long long*)(err + 2), (unsigned long long)dropped);
+        }
+    }
+};
+
+// Regroups particle i's processed friends (circles of the 0.6 prefix, which
+// contains the 0.5 prefix) from its circle segments into the sweep quads.
+// Run by the thread that has just written the segments, so they are read
+// back from cache.
+__device__ __forceinline__ void pack_friends(uint32_t i, uint32_t packed, const uint32_t* __restrict__ nb_list,
+                                             uint4* __restrict__ quads) {
+    int m;
+    const int L = circles_processed<3, 5>(packed, &m);
+    uint32_t acc[4] = {0u, 0u, 0u, 0u};
+    int na = 0, qo = 0;
+    for (int q = 0; q < L; ++q) {
+        const int cq = circle_count(packed, q);
+        const uint32_t* seg = nb_list + friend_seg(i, q);
+        for (int f = 0; f < cq; f += 4) {
+            const uint4 jj = *reinterpret_cast<const uint4*>(seg + f);
+            const uint32_t v4[4] = {jj.x, jj.y, jj.z, jj.w};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (f + t < cq) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) acc[u] = (u == na) ? v4[t] : acc[u];
+                    if (++na == 4) {
+                        quads[sweep_quad(i, qo++)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
+                        na = 0;
+                    }
+                }
+            }
+        }
+    }
+    if (na) quads[sweep_quad(i, qo)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
+}
+
+// Scans the global bucket [b.x, b.y] for particle i, four candidates at a time.
+__device__ __forceinline__ void scan_bucket_global(const float4& p, uint32_t i, uint2 b, uint32_t n,
+                                                   const float4* __restrict__ xs, const SceneConsts& c,
+                                                   uint32_t* __restrict__ nb_list, FriendWriter& w) {
+    if (b.x == kEmpty) return;
+    const uint32_t e = min(b.y, n - 1);
+    uint32_t j = b.x;
+    for (; j + 3 <= e; j += 4) {
+        const float4 q0 = xs[j], q1 = xs[j + 1], q2 = xs[j + 2], q3 = xs[j + 3];
+        const int c0 = friend_circle(p, q0, c), c1 = friend_circle(p, q1, c);
+        const int c2 = friend_circle(p, q2, c), c3 = friend_circle(p, q3, c);
+        w.take(i, j, c0, nb_list); w.take(i, j + 1, c1, nb_list);
+        w.take(i, j + 2, c2, nb_list); w.take(i, j + 3, c3, nb_list);
+    }
+    for (; j <= e; ++j) w.take(i, j, friend_circle(p, xs[j], c), nb_list);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __restrict__ scenes,
+                                                      const uint4* __restrict__ blocks, const float4* __restrict__ xs,
+                                                      const uint2* __restrict__ cells, uint32_t* __restrict__ nb_cnt,
+                                                      uint32_t* __restrict__ nb_list, uint4* __restrict__ quads,
+                                                      uint32_t n, uint32_t* err) {
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t i = be.y + threadIdx.x;
+    if (i >= be.z) return;
     const float4 p = xs[i];
     const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
-    uint32_t packed = 0, dropped = 0;
+    FriendWriter w;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz)
+                scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, nb_list, w);
+    w.finish(i, nb_cnt, err);
+    pack_friends(i, w.packed, nb_list, quads);
+}
+
+// Staging budget of the LDS variant per workgroup size: box cells, particles.
+template <int BS> struct FriendsBudget;
+template <> struct FriendsBudget<64> { static constexpr int cells = 128, staged = 1024; };
+template <> struct FriendsBudget<128> { static constexpr int cells = 256, staged = 1536; };
+template <> struct FriendsBudget<256> { static constexpr int cells = 512, staged = 2560; };
+template <> struct FriendsBudget<512> { static constexpr int cells = 1024, staged = 4096; };
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
+                                               const float4* __restrict__ xs, const uint2* __restrict__ cells,
+                                               uint32_t* __restrict__ nb_cnt, uint32_t* __restrict__ nb_list,
+                                               uint4* __restrict__ quads, uint32_t n, uint32_t* err) {
+    constexpr int kWaves = BS / 64;
+    constexpr int kMaxCells = FriendsBudget<BS>::cells;
+    constexpr int kMaxStaged = FriendsBudget<BS>::staged;
+    static_assert(kMaxCells <= 2 * BS, "the cell scan gives each thread two cells");
+    __shared__ float4 s_pos[kMaxStaged];
+    __shared__ uint32_t s_start[kMaxCells];
+    __shared__ uint32_t s_off[kMaxCells + 1];
+    __shared__ int s_red[6][kWaves];
+    __shared__ uint32_t s_scan[kWaves];
+
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const uint32_t i = be.y + tid;
+    const bool valid = i < be.z;
+    const float4 p = valid ? xs[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
+
+    // cell box of the tile
+    const int v0 = wave_min(valid ? cx : INT32_MAX), v1 = wave_min(valid ? cy : INT32_MAX);
+    const int v2 = wave_min(valid ? cz : INT32_MAX), v3 = wave_max(valid ? cx : INT32_MIN);
+    const int v4 = wave_max(valid ? cy : INT32_MIN), v5 = wave_max(valid ? cz : INT32_MIN);
+    if (lane == 0) {
+        s_red[0][wv] = v0; s_red[1][wv] = v1; s_red[2][wv] = v2;
+        s_red[3][wv] = v3; s_red[4][wv] = v4; s_red[5][wv] = v5;
+    }
+    __syncthreads();
+    int lo0 = INT32_MAX, lo1 = INT32_MAX, lo2 = INT32_MAX, hi0 = INT32_MIN, hi1 = INT32_MIN, hi2 = INT32_MIN;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) {
+        lo0 = min(lo0, s_red[0][q]); lo1 = min(lo1, s_red[1][q]); lo2 = min(lo2, s_red[2][q]);
+        hi0 = max(hi0, s_red[3][q]); hi1 = max(hi1, s_red[4][q]); hi2 = max(hi2, s_red[5][q]);
+    }
+    const int64_t dxs = (int64_t)hi0 - lo0 + 3, dys = (int64_t)hi1 - lo1 + 3, dzs = (int64_t)hi2 - lo2 + 3;
+    const int64_t ncell64 = dxs * dys * dzs;
+    bool staged = ncell64 <= kMaxCells;
+    const int ncell = staged ? (int)ncell64 : 0;
+    const int bx = lo0 - 1, by = lo1 - 1, bz = lo2 - 1;
+    const int ddx = (int)dxs, ddy = (int)dys;
+
+    if (staged) {
+        // bucket of every box cell; counts in s_off, prefix-summed below
+        for (int q = tid; q < ncell; q += BS) {
+            const int ix = q % ddx, iy = (q / ddx) % ddy, iz = q / (ddx * ddy);
+            const uint2 b = cells[cell_hash(bx + ix, by + iy, bz + iz, c)];
+            uint32_t cnt = 0;
+            if (b.x != kEmpty) {
+                const uint32_t e = min(b.y, n - 1);
+                cnt = e >= b.x ? e - b.x + 1 : 0;
+            }
+            s_start[q] = b.x;
+            s_off[q] = cnt;
+        }
+        __syncthreads();
+        // exclusive scan of s_off[0..ncell): each thread owns 2 consecutive cells
+        const int q0 = 2 * tid, q1 = 2 * tid + 1;
+        const uint32_t a0 = q0 < ncell ? s_off[q0] : 0u, a1 = q1 < ncell ? s_off[q1] : 0u;
+        uint32_t incl = a0 + a1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) s_scan[wv] = incl;
+        __syncthreads();
+        uint32_t base = 0, total = 0;
+        for (int q = 0; q < kWaves; ++q) {
+            if (q < wv) base += s_scan[q];
+            total += s_scan[q];
+        }
+        const uint32_t excl = base + incl - (a0 + a1);
+        if (q0 < ncell) s_off[q0] = excl;
+        if (q1 < ncell) s_off[q1] = excl + a0;
+        if (tid == 0) s_off[ncell] = total;
+        staged = total <= (uint32_t)kMaxStaged;
+        __syncthreads();
+        if (staged) {
+            for (uint32_t s = tid; s < total; s += BS) {
+                // box cell holding slot s: the last cell whose offset is <= s
+                int lo_c = 0, hi_c = ncell - 1;
+                while (lo_c < hi_c) {
+                    const int mid = (lo_c + hi_c + 1) >> 1;
+                    if (s_off[mid] <= s) lo_c = mid; else hi_c = mid - 1;
+                }
+                s_pos[s] = xs[s_start[lo_c] + (s - s_off[lo_c])];
+            }
+            __syncthreads();
+        }
+    }
+    if (!staged && tid == 0) atomicAdd(err + 4, 1u);
+    if (!valid) return;
+
+    FriendWriter w;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
-                const uint2 b = cells[cell_hash(cx + dx, cy + dy, cz + dz, c)];
-                if (b.x == kEmpty) continue;
-                const uint32_t e = min(b.y, c.n - 1);
-                for (uint32_t j = b.x; j <= e; ++j) {
-                    if (j == i) continue;
-                    const float4 q = xs[j];
-                    const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-                    const float r2 = rx * rx + ry * ry + rz * rz;
-                    if (r2 >= c.h2) continue;
-                    const float a = fmaxf(0.0f, (sqrtf(r2) - c.min_r) / c.span);
-                    const int circ = min((int)(a * a * a * (float)kCircles), kCircles - 1);
-                    const uint32_t cnt = (packed >> (6 * circ)) & 63u;
-                    if (cnt >= (uint32_t)kPerCircle) { ++dropped; continue; }
-                    nb_idx[friend_slot(circ, (int)cnt, i, c.n)] = j;
-                    packed += 1u << (6 * circ);
+                if (staged) {
+                    const int q = (cx + dx - bx) + ddx * ((cy + dy - by) + ddy * (cz + dz - bz));
+                    const uint32_t off = s_off[q], cnt = s_off[q + 1] - off, gs = s_start[q];
+                    uint32_t k = 0;
+                    for (; k + 4 <= cnt; k += 4) {
+                        const float4 q0 = s_pos[off + k], q1 = s_pos[off + k + 1];
+                        const float4 q2 = s_pos[off + k + 2], q3 = s_pos[off + k + 3];
+                        const int c0 = friend_circle(p, q0, c), c1 = friend_circle(p, q1, c);
+                        const int c2 = friend_circle(p, q2, c), c3 = friend_circle(p, q3, c);
+                        w.take(i, gs + k, c0, nb_list); w.take(i, gs + k + 1, c1, nb_list);
+                        w.take(i, gs + k + 2, c2, nb_list); w.take(i, gs + k + 3, c3, nb_list);
+                    }
+                    for (; k < cnt; ++k) w.take(i, gs + k, friend_circle(p, s_pos[off + k], c), nb_list);
+                } else {
+                    scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, nb_list, w);
                 }
             }
-    nb_cnt[i] = packed;
-    if (dropped) {
-        atomicOr(err, (uint32_t)PBF_FLAG_FRIENDS_OVERFLOW);
-        atomicAdd((unsigned long long*)(err + 2), (unsigned long long)dropped);
+    w.finish(i, nb_cnt, err);
+    pack_friends(i, w.packed, nb_list, quads);
+}
+
+// ---------------------------------------------------------------------------
+// Friend sweeps.  Each walks the friends of the circles it processes in the
+// reference's order (circle-major, then scan order), four per 16-byte load:
+// the four gathers are issued together, then the updates run in order.
+// ---------------------------------------------------------------------------
+
+template <int L_NUM, int L_DEN, class Fn4, class Fn1>
+__device__ __forceinline__ void for_friends(uint32_t i, uint32_t packed, const uint4* __restrict__ quads, Fn4&& fn4,
+                                            Fn1&& fn1) {
+    int m;
+    (void)circles_processed<L_NUM, L_DEN>(packed, &m);
+    int q = 0;
+    for (; 4 * q + 4 <= m; ++q) fn4(quads[sweep_quad(i, q)]);
+    const int rest = m - 4 * q;
+    if (rest > 0) {
+        const uint4 jj = quads[sweep_quad(i, q)];
+        fn1(jj.x);
+        if (rest > 1) fn1(jj.y);
+        if (rest > 2) fn1(jj.z);
     }
 }
 
 // compute_scaling.cl:1-113 + pack_data.cl (lambda into x*.w, in place: no
 // thread of this kernel reads another particle's w).
-__global__ __launch_bounds__(kThreads) void k_lambda(float4* __restrict__ xs, const uint32_t* __restrict__ nb_cnt,
-                                                    const uint32_t* __restrict__ nb_idx, float* __restrict__ rho,
-                                                    float* __restrict__ lambda, StepConsts c) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= c.n) return;
-    const float4 p = xs[i];
-    const uint32_t packed = nb_cnt[i];
-    const int L = circles_processed<3, 5>(packed);
+struct LambdaAcc {
     float dens = 0.0f, gsum = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    for (int q = 0; q < L; ++q) {
-        const int cq = circle_count(packed, q);
-        for (int f = 0; f < cq; ++f) {
-            const uint32_t j = nb_idx[friend_slot(q, f, i, c.n)];
-            const float4 pj = xs[j];
-            const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 < c.h2) {
-                const float rl = sqrtf(r2);
-                const float hr = c.h - rl;
-                const float s = c.spiky * hr * hr;
-                const float ax = s * rx / rl, ay = s * ry / rl, az = s * rz / rl;
-                const float d = c.h2 - r2;
-                dens += d * d * d;
-                gsum += ax * ax + ay * ay + az * az;
-                gx += ax; gy += ay; gz += az;
-            }
+    __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c) {
+        const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 < c.h2) {
+            const float rl = sqrtf(r2);
+            const float hr = c.h - rl;
+            const float s = c.spiky * hr * hr;
+            const float ax = s * rx / rl, ay = s * ry / rl, az = s * rz / rl;
+            const float d = c.h2 - r2;
+            dens += d * d * d;
+            gsum += ax * ax + ay * ay + az * az;
+            gx += ax; gy += ay; gz += az;
         }
     }
-    dens *= c.poly6;
+};
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
+                                              float4* __restrict__ xs, const uint32_t* __restrict__ nb_cnt,
+                                              const uint4* __restrict__ nb_list, float* __restrict__ rho,
+                                              float* __restrict__ lambda, uint32_t n) {
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t i = be.y + threadIdx.x;
+    if (i >= be.z) return;
+    const float4 p = xs[i];
+    LambdaAcc a;
+    for_friends<3, 5>(
+        i, nb_cnt[i], nb_list,
+        [&](const uint4& jj) {
+            const float4 q0 = xs[jj.x], q1 = xs[jj.y], q2 = xs[jj.z], q3 = xs[jj.w];
+            a.add(p, q0, c); a.add(p, q1, c); a.add(p, q2, c); a.add(p, q3, c);
+        },
+        [&](uint32_t j) { a.add(p, xs[j], c); });
+    const float dens = a.dens * c.poly6;
     rho[i] = dens;
-    gsum += gx * gx + gy * gy + gz * gz;
+    const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
     const float C = (dens / c.rest_density) - 1.0f;
     const float lam = -1.0f * C / (gsum / (c.rest_density * c.rest_density) + c.epsilon);
     lambda[i] = lam;
@@ -209,41 +478,47 @@ __global__ __launch_bounds__(kThreads) void k_lambda(float4* __restrict__ xs, co
 
 // compute_delta.cl:44-186 + update_predicted.cl; on the last iteration also
 // pack_data.cl (density into x*.w) and update_velocities.cl.
-template <bool LAST>
-__global__ __launch_bounds__(kThreads) void k_delta(const float4* __restrict__ xs, float4* __restrict__ xs_next,
-                                                   float4* __restrict__ x, float4* __restrict__ v,
-                                                   const float* __restrict__ rho,
-                                                   const uint32_t* __restrict__ nb_cnt,
-                                                   const uint32_t* __restrict__ nb_idx, StepConsts c) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= c.n) return;
-    const float4 p = xs[i];
-    const uint32_t packed = nb_cnt[i];
-    const int L = circles_processed<1, 2>(packed);
+struct DeltaAcc {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (int q = 0; q < L; ++q) {
-        const int cq = circle_count(packed, q);
-        for (int f = 0; f < cq; ++f) {
-            const uint32_t j = nb_idx[friend_slot(q, f, i, c.n)];
-            const float4 pj = xs[j];
-            const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 < c.h2) {
-                const float rl = sqrtf(r2);
-                const float hr = c.h - rl;
-                const float ax = rx / rl * hr * hr, ay = ry / rl * hr * hr, az = rz / rl * hr * hr;
-                const float d = c.h2 - r2;
-                const float p6 = d * d * d;
-                const float ratio = p6 / c.poly6_q;
-                const float scorr = c.st_k * ratio * ratio * ratio * ratio;
-                const float coef = p.w + pj.w + scorr;
-                sx += coef * ax; sy += coef * ay; sz += coef * az;
-            }
+    __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c) {
+        const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 < c.h2) {
+            const float rl = sqrtf(r2);
+            const float hr = c.h - rl;
+            const float ax = rx / rl * hr * hr, ay = ry / rl * hr * hr, az = rz / rl * hr * hr;
+            const float d = c.h2 - r2;
+            const float p6 = d * d * d;
+            const float ratio = p6 / c.poly6_q;
+            const float scorr = c.st_k * ratio * ratio * ratio * ratio;
+            const float coef = p.w + pj.w + scorr;
+            sx += coef * ax; sy += coef * ay; sz += coef * az;
         }
     }
-    float fx = p.x + (-c.spiky * sx) / c.rest_density;
-    float fy = p.y + (-c.spiky * sy) / c.rest_density;
-    float fz = p.z + (-c.spiky * sz) / c.rest_density;
+};
+
+template <int BS, bool LAST>
+__global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
+                                             const float4* __restrict__ xs, float4* __restrict__ xs_next,
+                                             float4* __restrict__ x, float4* __restrict__ v,
+                                             const float* __restrict__ rho, const uint32_t* __restrict__ nb_cnt,
+                                             const uint4* __restrict__ nb_list, uint32_t n) {
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t i = be.y + threadIdx.x;
+    if (i >= be.z) return;
+    const float4 p = xs[i];
+    DeltaAcc a;
+    for_friends<1, 2>(
+        i, nb_cnt[i], nb_list,
+        [&](const uint4& jj) {
+            const float4 q0 = xs[jj.x], q1 = xs[jj.y], q2 = xs[jj.z], q3 = xs[jj.w];
+            a.add(p, q0, c); a.add(p, q1, c); a.add(p, q2, c); a.add(p, q3, c);
+        },
+        [&](uint32_t j) { a.add(p, xs[j], c); });
+    float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
+    float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
+    float fz = p.z + (-c.spiky * a.sz) / c.rest_density;
     const float4 prev = x[i];
     if (c.walls) apply_walls(fx, fy, fz, prev, c);
     const float dx = fx - p.x, dy = fy - p.y, dz = fz - p.z;
@@ -267,77 +542,94 @@ __global__ __launch_bounds__(kThreads) void k_delta(const float4* __restrict__ x
 }
 
 // apply_viscosity.cl:1-87, Jacobi: v_j from the snapshot v_in.
-__global__ __launch_bounds__(kThreads) void k_viscosity(const float4* __restrict__ xs, const float4* __restrict__ v_in,
-                                                       float4* __restrict__ v_out, float4* __restrict__ omega,
-                                                       const uint32_t* __restrict__ nb_cnt,
-                                                       const uint32_t* __restrict__ nb_idx, StepConsts c) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= c.n) return;
-    const float4 p = xs[i];
-    const float4 vi = v_in[i];
-    const uint32_t packed = nb_cnt[i];
-    const int L = circles_processed<1, 2>(packed);
+struct ViscAcc {
     float vsx = 0.0f, vsy = 0.0f, vsz = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
-    for (int q = 0; q < L; ++q) {
-        const int cq = circle_count(packed, q);
-        for (int f = 0; f < cq; ++f) {
-            const uint32_t j = nb_idx[friend_slot(q, f, i, c.n)];
-            const float4 pj = xs[j];
-            const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 < c.h2 && fabsf(pj.w) > 1e-8f) {
-                const float4 vj = v_in[j];
-                const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-                const float d = c.h2 - r2;
-                const float rl = sqrtf(r2);
-                const float hr = c.h - rl;
-                const float ax = rx / rl * hr * hr, ay = ry / rl * hr * hr, az = rz / rl * hr * hr;
-                ox += uy * az - uz * ay;
-                oy += uz * ax - ux * az;
-                oz += ux * ay - uy * ax;
-                const float p6 = d * d * d;
-                vsx += ux * p6; vsy += uy * p6; vsz += uz * p6;
-            }
+    __device__ __forceinline__ void add(const float4& p, const float4& vi, const float4& pj, const float4& vj,
+                                        const SceneConsts& c) {
+        const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 < c.h2 && fabsf(pj.w) > 1e-8f) {
+            const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+            const float d = c.h2 - r2;
+            const float rl = sqrtf(r2);
+            const float hr = c.h - rl;
+            const float ax = rx / rl * hr * hr, ay = ry / rl * hr * hr, az = rz / rl * hr * hr;
+            ox += uy * az - uz * ay;
+            oy += uz * ax - ux * az;
+            oz += ux * ay - uy * ax;
+            const float p6 = d * d * d;
+            vsx += ux * p6; vsy += uy * p6; vsz += uz * p6;
         }
     }
+};
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict__ scenes,
+                                                 const uint4* __restrict__ blocks, const float4* __restrict__ xs,
+                                                 const float4* __restrict__ v_in, float4* __restrict__ v_out,
+                                                 float4* __restrict__ omega, const uint32_t* __restrict__ nb_cnt,
+                                                 const uint4* __restrict__ nb_list, uint32_t n) {
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t i = be.y + threadIdx.x;
+    if (i >= be.z) return;
+    const float4 p = xs[i];
+    const float4 vi = v_in[i];
+    ViscAcc a;
+    for_friends<1, 2>(
+        i, nb_cnt[i], nb_list,
+        [&](const uint4& jj) {
+            const float4 q0 = xs[jj.x], q1 = xs[jj.y], q2 = xs[jj.z], q3 = xs[jj.w];
+            const float4 u0 = v_in[jj.x], u1 = v_in[jj.y], u2 = v_in[jj.z], u3 = v_in[jj.w];
+            a.add(p, vi, q0, u0, c); a.add(p, vi, q1, u1, c); a.add(p, vi, q2, u2, c); a.add(p, vi, q3, u3, c);
+        },
+        [&](uint32_t j) { a.add(p, vi, xs[j], v_in[j], c); });
     float4 vo;
-    vo.x = vi.x + c.viscosity * (vsx * c.poly6);
-    vo.y = vi.y + c.viscosity * (vsy * c.poly6);
-    vo.z = vi.z + c.viscosity * (vsz * c.poly6);
+    vo.x = vi.x + c.viscosity * (a.vsx * c.poly6);
+    vo.y = vi.y + c.viscosity * (a.vsy * c.poly6);
+    vo.z = vi.z + c.viscosity * (a.vsz * c.poly6);
     vo.w = vi.w + c.viscosity * 0.0f;
     v_out[i] = vo;
-    omega[i] = make_float4(ox * -c.spiky, oy * -c.spiky, oz * -c.spiky, 0.0f);
+    omega[i] = make_float4(a.ox * -c.spiky, a.oy * -c.spiky, a.oz * -c.spiky, 0.0f);
 }
 
 // apply_vorticity.cl:1-87
-__global__ __launch_bounds__(kThreads) void k_vorticity(const float4* __restrict__ xs, float4* __restrict__ v,
-                                                       const float4* __restrict__ omega,
-                                                       const uint32_t* __restrict__ nb_cnt,
-                                                       const uint32_t* __restrict__ nb_idx, StepConsts c) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= c.n) return;
-    const float4 p = xs[i];
-    const uint32_t packed = nb_cnt[i];
-    const int L = circles_processed<1, 2>(packed);
+struct VortAcc {
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    for (int q = 0; q < L; ++q) {
-        const int cq = circle_count(packed, q);
-        for (int f = 0; f < cq; ++f) {
-            const uint32_t j = nb_idx[friend_slot(q, f, i, c.n)];
-            const float4 pj = xs[j];
-            const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 < c.h2 && fabsf(pj.w) > 1e-8f) {
-                const float rl = sqrtf(r2);
-                const float hr = c.h - rl;
-                const float ax = rx / rl * hr * hr, ay = ry / rl * hr * hr, az = rz / rl * hr * hr;
-                const float4 oj = omega[j];
-                const float wl = sqrtf(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
-                ex += wl * ax; ey += wl * ay; ez += wl * az;
-            }
+    __device__ __forceinline__ void add(const float4& p, const float4& pj, const float4& oj, const SceneConsts& c) {
+        const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 < c.h2 && fabsf(pj.w) > 1e-8f) {
+            const float rl = sqrtf(r2);
+            const float hr = c.h - rl;
+            const float ax = rx / rl * hr * hr, ay = ry / rl * hr * hr, az = rz / rl * hr * hr;
+            const float wl = sqrtf(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
+            ex += wl * ax; ey += wl * ay; ez += wl * az;
         }
     }
-    ex *= -c.spiky; ey *= -c.spiky; ez *= -c.spiky;
+};
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict__ scenes,
+                                                 const uint4* __restrict__ blocks, const float4* __restrict__ xs,
+                                                 float4* __restrict__ v, const float4* __restrict__ omega,
+                                                 const uint32_t* __restrict__ nb_cnt,
+                                                 const uint4* __restrict__ nb_list, uint32_t n) {
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t i = be.y + threadIdx.x;
+    if (i >= be.z) return;
+    const float4 p = xs[i];
+    VortAcc a;
+    for_friends<1, 2>(
+        i, nb_cnt[i], nb_list,
+        [&](const uint4& jj) {
+            const float4 q0 = xs[jj.x], q1 = xs[jj.y], q2 = xs[jj.z], q3 = xs[jj.w];
+            const float4 o0 = omega[jj.x], o1 = omega[jj.y], o2 = omega[jj.z], o3 = omega[jj.w];
+            a.add(p, q0, o0, c); a.add(p, q1, o1, c); a.add(p, q2, o2, c); a.add(p, q3, o3, c);
+        },
+        [&](uint32_t j) { a.add(p, xs[j], omega[j], c); });
+    float ex = a.ex * -c.spiky, ey = a.ey * -c.spiky, ez = a.ez * -c.spiky;
     const float l = sqrtf(ex * ex + ey * ey + ez * ez);
     if (l > 0.0f) { ex /= l; ey /= l; ez /= l; }
     const float4 oi = omega[i];
@@ -351,24 +643,32 @@ __global__ __launch_bounds__(kThreads) void k_vorticity(const float4* __restrict
     v[i] = vi;
 }
 
-inline unsigned blocks_for(uint32_t n, uint32_t per_block) { return (unsigned)((n + per_block - 1) / per_block); }
-
 }  // namespace
+
+// Launch a kernel template at the workgroup size chosen for its family.
+#define PBF_DISPATCH_BS(LOG2, LAUNCH)                        \
+    switch (LOG2) {                                          \
+        case 6: { constexpr int BS = 64; LAUNCH; } break;    \
+        case 7: { constexpr int BS = 128; LAUNCH; } break;   \
+        case 9: { constexpr int BS = 512; LAUNCH; } break;   \
+        default: { constexpr int BS = 256; LAUNCH; } break;  \
+    }
 
 hipError_t launch_predict_hash(pbf_ctx* ctx, bool reset_old) {
     track_begin(ctx, "predictPositions");
-    hipLaunchKernelGGL(k_predict_hash, dim3(blocks_for(ctx->n, kThreads * kPredictItems)), dim3(kThreads), 0,
-                       ctx->stream, ctx->x, ctx->v, ctx->keys_in, ctx->keys_sorted,
-                       reinterpret_cast<uint2*>(ctx->cells), reset_old ? 1u : 0u, ctx->sort.hist, ctx->sort.passes,
-                       ctx->pause_now, ctx->c, ctx->err);
+    hipLaunchKernelGGL(k_predict_hash, dim3(ctx->bt_predict.count), dim3(kPredictThreads), 0, ctx->stream, ctx->sc_d,
+                       ctx->bt_predict.d, ctx->x, ctx->v, ctx->keys_in, ctx->keys_sorted,
+                       reinterpret_cast<uint2*>(ctx->cells), reset_old ? 1u : 0u, ctx->total_cells, ctx->sort.hist,
+                       ctx->sort.passes, ctx->pause_now, ctx->err);
     track_end(ctx);
     return hipGetLastError();
 }
 
 hipError_t launch_predict_only(pbf_ctx* ctx, uint32_t pause) {
     track_begin(ctx, "predictPositions");
-    hipLaunchKernelGGL(k_predict_only, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream, ctx->x,
-                       ctx->v, ctx->xs[0], pause, ctx->c);
+    const BlockTable& t = ctx->bt[2];
+    hipLaunchKernelGGL(k_predict_only, dim3(t.count), dim3(256), 0, ctx->stream, ctx->sc_d, t.d, ctx->x, ctx->v,
+                       ctx->xs[0], pause);
     track_end(ctx);
     ctx->xs_cur = 0;
     return hipGetLastError();
@@ -376,37 +676,50 @@ hipError_t launch_predict_only(pbf_ctx* ctx, uint32_t pause) {
 
 hipError_t launch_gather(pbf_ctx* ctx) {
     track_begin(ctx, "sortParticles");
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream, ctx->perm,
-                       ctx->x, ctx->v, ctx->x_alt, ctx->xs[0], ctx->pause_now, ctx->c, ctx->err);
+    const BlockTable& t = ctx->bt[2];
+    hipLaunchKernelGGL(k_gather, dim3(t.count), dim3(256), 0, ctx->stream, ctx->sc_d, t.d, ctx->perm, ctx->x, ctx->v,
+                       ctx->x_alt, ctx->xs[0], ctx->pause_now, ctx->n, ctx->err);
     track_end(ctx);
-    float4* t = ctx->x;
+    float4* tmp = ctx->x;
     ctx->x = ctx->x_alt;
-    ctx->x_alt = t;
+    ctx->x_alt = tmp;
     ctx->xs_cur = 0;
     return hipGetLastError();
 }
 
 hipError_t launch_cells(pbf_ctx* ctx) {
     track_begin(ctx, "updateCells");
-    hipLaunchKernelGGL(k_cells, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                       ctx->keys_sorted, ctx->cells, ctx->n, ctx->c.gbs, ctx->err);
+    hipLaunchKernelGGL(k_cells, dim3((ctx->n + 255) / 256), dim3(256), 0, ctx->stream, ctx->keys_sorted, ctx->cells,
+                       ctx->n, ctx->total_cells, ctx->err);
     track_end(ctx);
     return hipGetLastError();
 }
 
 hipError_t launch_friends(pbf_ctx* ctx) {
     track_begin(ctx, "buildFriendsList");
-    hipLaunchKernelGGL(k_friends, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                       ctx->xs[ctx->xs_cur], reinterpret_cast<const uint2*>(ctx->cells), ctx->nb_cnt, ctx->nb_idx,
-                       ctx->c, ctx->err);
+    const BlockTable& t = table_for(ctx, F_FRIENDS);
+    if (ctx->friends_staged) {
+        PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS],
+                        hipLaunchKernelGGL(k_friends<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d,
+                                           ctx->xs[ctx->xs_cur], reinterpret_cast<const uint2*>(ctx->cells),
+                                           ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, ctx->err));
+    } else {
+        PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS],
+                        hipLaunchKernelGGL(k_friends_direct<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
+                                           t.d, ctx->xs[ctx->xs_cur], reinterpret_cast<const uint2*>(ctx->cells),
+                                           ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, ctx->err));
+    }
     track_end(ctx);
     return hipGetLastError();
 }
 
 hipError_t launch_lambda(pbf_ctx* ctx, int iter) {
     track_begin(ctx, "computeScaling", iter);
-    hipLaunchKernelGGL(k_lambda, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                       ctx->xs[ctx->xs_cur], ctx->nb_cnt, ctx->nb_idx, ctx->rho, ctx->lambda, ctx->c);
+    const BlockTable& t = table_for(ctx, F_LAMBDA);
+    PBF_DISPATCH_BS(ctx->bs_log2[F_LAMBDA],
+                    hipLaunchKernelGGL(k_lambda<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d,
+                                       ctx->xs[ctx->xs_cur], ctx->nb_cnt, ctx->nb_quads, ctx->rho, ctx->lambda,
+                                       ctx->n));
     track_end(ctx);
     return hipGetLastError();
 }
@@ -414,14 +727,18 @@ hipError_t launch_lambda(pbf_ctx* ctx, int iter) {
 hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last) {
     track_begin(ctx, "computeDelta", iter);
     const int cur = ctx->xs_cur;
-    if (last)
-        hipLaunchKernelGGL(k_delta<true>, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                           ctx->xs[cur], ctx->xs[1 - cur], ctx->x, ctx->v, ctx->rho, ctx->nb_cnt, ctx->nb_idx,
-                           ctx->c);
-    else
-        hipLaunchKernelGGL(k_delta<false>, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                           ctx->xs[cur], ctx->xs[1 - cur], ctx->x, ctx->v, ctx->rho, ctx->nb_cnt, ctx->nb_idx,
-                           ctx->c);
+    const BlockTable& t = table_for(ctx, F_DELTA);
+    if (last) {
+        PBF_DISPATCH_BS(ctx->bs_log2[F_DELTA],
+                        hipLaunchKernelGGL((k_delta<BS, true>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
+                                           t.d, ctx->xs[cur], ctx->xs[1 - cur], ctx->x, ctx->v, ctx->rho, ctx->nb_cnt,
+                                           ctx->nb_quads, ctx->n));
+    } else {
+        PBF_DISPATCH_BS(ctx->bs_log2[F_DELTA],
+                        hipLaunchKernelGGL((k_delta<BS, false>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
+                                           t.d, ctx->xs[cur], ctx->xs[1 - cur], ctx->x, ctx->v, ctx->rho, ctx->nb_cnt,
+                                           ctx->nb_quads, ctx->n));
+    }
     track_end(ctx);
     ctx->xs_cur = 1 - cur;
     return hipGetLastError();
@@ -429,19 +746,24 @@ hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last) {
 
 hipError_t launch_viscosity(pbf_ctx* ctx) {
     track_begin(ctx, "applyViscosity");
-    hipLaunchKernelGGL(k_viscosity, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                       ctx->xs[ctx->xs_cur], ctx->v, ctx->v_alt, ctx->omega, ctx->nb_cnt, ctx->nb_idx, ctx->c);
+    const BlockTable& t = table_for(ctx, F_VISC);
+    PBF_DISPATCH_BS(ctx->bs_log2[F_VISC],
+                    hipLaunchKernelGGL(k_viscosity<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d,
+                                       ctx->xs[ctx->xs_cur], ctx->v, ctx->v_alt, ctx->omega, ctx->nb_cnt,
+                                       ctx->nb_quads, ctx->n));
     track_end(ctx);
-    float4* t = ctx->v;
+    float4* tmp = ctx->v;
     ctx->v = ctx->v_alt;
-    ctx->v_alt = t;
+    ctx->v_alt = tmp;
     return hipGetLastError();
 }
 
 hipError_t launch_vorticity(pbf_ctx* ctx) {
     track_begin(ctx, "applyVorticity");
-    hipLaunchKernelGGL(k_vorticity, dim3(blocks_for(ctx->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                       ctx->xs[ctx->xs_cur], ctx->v, ctx->omega, ctx->nb_cnt, ctx->nb_idx, ctx->c);
+    const BlockTable& t = table_for(ctx, F_VORT);
+    PBF_DISPATCH_BS(ctx->bs_log2[F_VORT],
+                    hipLaunchKernelGGL(k_vorticity<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d,
+                                       ctx->xs[ctx->xs_cur], ctx->v, ctx->omega, ctx->nb_cnt, ctx->nb_quads, ctx->n));
     track_end(ctx);
     return hipGetLastError();
 }

@@ -20,8 +20,23 @@ constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kMaxPasses = 4;
 
-// Device error words: [0] PBF_FLAG_* bits, [1..2] dropped-friend count (u64).
-constexpr int kErrWords = 4;
+// Device error words: [0] PBF_FLAG_* bits, [2..3] dropped-friend count (u64),
+// [4] tiles whose neighbourhood did not fit in LDS (diagnostic).
+constexpr int kErrWords = 8;
+
+// Per-particle kernels run over a block table: one uint4 {scene, first,
+// end, 0} per workgroup, so a workgroup never straddles two scenes.  Tables
+// exist for these workgroup sizes (index = log2(size) - 6).
+constexpr int kNumBlockSizes = 5;  // 64, 128, 256, 512, 1024
+constexpr int kPredictItems = 4;   // particles per thread in k_predict_hash
+
+struct BlockTable {
+    uint4* d = nullptr;
+    uint32_t count = 0;
+};
+
+// Kernel families whose workgroup size the autotuner chooses.
+enum Family { F_FRIENDS = 0, F_LAMBDA, F_DELTA, F_VISC, F_VORT, F_COUNT };
 
 struct SortBuffers {
     uint32_t* ws = nullptr;       // [hist P*256][ctr 64][status P*tiles*256]
@@ -44,15 +59,34 @@ struct EventPair {
     int tracker = -1;
 };
 
+// One instantiated substep graph per variant of the buffer rotation; the
+// host applies the rotation the captured step performs when it replays it.
+struct GraphEntry {
+    uint32_t key = 0;
+    bool sorts = false;
+    hipGraphExec_t exec = nullptr;
+    float4 *post_x = nullptr, *post_x_alt = nullptr, *post_v = nullptr, *post_v_alt = nullptr;
+    int post_xs_cur = 0;
+};
+
 }  // namespace pbf
 
 struct pbf_ctx {
-    pbf_params P{};
-    pbf_options opt{};
-    pbf::StepConsts c{};
-    uint32_t n = 0;
+    std::vector<pbf_params> P;        // per scene
+    std::vector<pbf_options> opt;     // per scene
+    std::vector<pbf::SceneConsts> sc; // per scene (host copy)
+    pbf::SceneConsts* sc_d = nullptr;
+    uint32_t S = 0;                   // scenes
+    uint32_t n = 0;                   // particles, all scenes
+    uint32_t iters = 0;
+    uint32_t total_cells = 0;
     int device = 0;
     hipStream_t stream = nullptr;
+
+    pbf::BlockTable bt[pbf::kNumBlockSizes];
+    pbf::BlockTable bt_predict;       // 256*kPredictItems particles per block
+    int bs_log2[pbf::F_COUNT] = {8, 8, 8, 8, 8};  // chosen workgroup sizes
+    bool friends_staged = false;      // friends build: LDS-staged candidates or global reads
 
     // particle arrays (current cell-sorted order)
     float4* x = nullptr;        // positions, w = |v|
@@ -75,16 +109,22 @@ struct pbf_ctx {
     pbf::SortBuffers sort;
 
     // grid and friends
-    uint32_t* cells = nullptr;       // uint2[gbs]
+    uint32_t* cells = nullptr;       // uint2[total_cells]
     uint32_t* nb_cnt = nullptr;      // packed 5x6-bit circle counts
-    uint32_t* nb_idx = nullptr;      // [5][50][n]
+    uint32_t* nb_list = nullptr;     // friends, [n][5 circles][kSeg]
+    uint4* nb_quads = nullptr;       // processed friends, sweep quads
 
     uint32_t* err = nullptr;
 
     uint32_t pause_now = 0;     // pause flag of the step being enqueued
     bool have_state = false;
-    bool have_sorted = false;
+    bool cells_hold_keys = false;  // cells hold the entries of keys_sorted (reset them first)
+    bool order_sorted = false;     // particle order == keys_sorted order (pause may skip the sort)
+    float4* x_base = nullptr;      // buffer identities for the graph keys
+    float4* v_base = nullptr;
     bool profiling = false;
+    bool use_graph = false;
+    std::vector<pbf::GraphEntry> graphs;
     std::vector<pbf::Tracker> trackers;
     std::vector<pbf::EventPair> events;  // recorded (start, stop) pairs
     size_t events_used = 0;
@@ -94,7 +134,7 @@ struct pbf_ctx {
 namespace pbf {
 // Launchers (pbf_forward.hip, pbf_sort.hip).  All enqueue on ctx->stream.
 hipError_t launch_clear(pbf_ctx* ctx);
-hipError_t launch_predict_hash(pbf_ctx* ctx, bool reset_old_cells);
+hipError_t launch_predict_hash(pbf_ctx* ctx, bool reset_old);
 hipError_t launch_predict_only(pbf_ctx* ctx, uint32_t pause);
 hipError_t launch_sort(pbf_ctx* ctx);
 hipError_t launch_gather(pbf_ctx* ctx);
@@ -109,4 +149,6 @@ hipError_t launch_vorticity(pbf_ctx* ctx);
 void track_begin(pbf_ctx* ctx, const char* name, int index = -1);
 void track_end(pbf_ctx* ctx);
 void collect_timings(pbf_ctx* ctx);
+
+inline const BlockTable& table_for(const pbf_ctx* ctx, Family f) { return ctx->bt[ctx->bs_log2[f] - 6]; }
 }  // namespace pbf

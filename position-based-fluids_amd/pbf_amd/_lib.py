@@ -101,6 +101,23 @@ class PbfDevBufs(ctypes.Structure):
     ]
 
 
+class PbfStats(ctypes.Structure):
+    _fields_ = [
+        ("particles", ctypes.c_uint32),
+        ("scenes", ctypes.c_uint32),
+        ("total_cells", ctypes.c_uint32),
+        ("sort_passes", ctypes.c_uint32),
+        ("sort_tiles", ctypes.c_uint32),
+        ("unstaged_tiles", ctypes.c_uint32),
+        ("workgroup_size", ctypes.c_uint32 * 5),
+        ("friends_staged", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 7),
+    ]
+
+
+FAMILIES = ("friends", "lambda", "delta", "viscosity", "vorticity")
+
+
 class PbfError(RuntimeError):
     """A PBF_E_* return code; the message is pbf_last_error()."""
 
@@ -125,6 +142,18 @@ _SIGNATURES = {
     "pbf_params_finalize": (None, [ctypes.POINTER(PbfParams)]),
     "pbf_load_parameters": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(PbfParams)]),
     "pbf_create": (ctypes.c_int, [ctypes.POINTER(PbfParams), ctypes.POINTER(PbfOptions), ctypes.POINTER(_VP)]),
+    "pbf_create_batch": (ctypes.c_int, [ctypes.POINTER(PbfParams), ctypes.POINTER(PbfOptions), ctypes.c_uint32,
+                                        ctypes.POINTER(_VP)]),
+    "pbf_scene_count": (ctypes.c_int, [_VP]),
+    "pbf_scene_offsets": (ctypes.c_int, [_VP, _UP]),
+    "pbf_set_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "pbf_reset_timings": (ctypes.c_int, [_VP]),
+    "pbf_set_workgroup_sizes": (ctypes.c_int, [_VP, _UP]),
+    "pbf_set_friends_mode": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "pbf_autotune": (ctypes.c_int, [_VP, ctypes.c_uint32, _UP]),
+    "pbf_stats": (ctypes.c_int, [_VP, ctypes.POINTER(PbfStats)]),
+    "pbf_friend_circle_thresholds": (ctypes.c_int, [ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+    "pbf_copy_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "pbf_set_state": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32]),
     "pbf_step": (ctypes.c_int, [_VP, ctypes.c_uint32]),
     "pbf_run": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32]),
@@ -155,6 +184,13 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = args
         _LIB = L
     return _LIB
+
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Sustained plain-copy bandwidth in GB/s (read + write bytes)."""
+    g = ctypes.c_double()
+    check(lib().pbf_copy_probe(nbytes, reps, ctypes.byref(g)))
+    return g.value
 
 
 def check(rc: int, ctx=None) -> None:

@@ -39,7 +39,7 @@ HIP_FLAGS = [
     f"-I{REPO_ROOT / 'include'}",
 ]
 
-SOURCES = ["pbf_sort.hip", "pbf_forward.hip", "pbf_capi.hip", "pbf_params.cpp"]
+SOURCES = ["pbf_sort.hip", "pbf_forward.hip", "pbf_capi.hip", "pbf_probe.hip", "pbf_params.cpp"]
 
 
 def hipcc() -> str:

@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import PbfDevBufs, PbfError, PbfOptions, PbfParams, PbfTiming, check, lib
+from ._lib import FAMILIES, PbfDevBufs, PbfError, PbfOptions, PbfParams, PbfStats, PbfTiming, check, lib
 from .scenes import default_options
 
 _CIRCLES = 5
@@ -85,12 +85,15 @@ class Simulation:
                                   _ptr(out["lambda"]), _ptr(out["omega"])), self._ctx)
         return out
 
+    def _total_cells(self) -> int:
+        return int(self.params.gridBufSize)
+
     def get_structure(self, friends: bool = True) -> dict:
         n = self.n
         out = {
             "keys": np.empty(n, np.uint32),
             "perm": np.empty(n, np.uint32),
-            "cells": np.empty(2 * int(self.params.gridBufSize), np.uint32),
+            "cells": np.empty(2 * self._total_cells(), np.uint32),
         }
         fr = np.empty(n * _CIRCLES * (1 + _PER_CIRCLE), np.uint32) if friends else None
         check(lib().pbf_get_structure(self._ctx, _ptr(out["keys"]), _ptr(out["perm"]), _ptr(out["cells"]), _ptr(fr)),
@@ -118,6 +121,34 @@ class Simulation:
             for t in arr[: cnt.value]
         }
 
+    def reset_timings(self):
+        check(lib().pbf_reset_timings(self._ctx), self._ctx)
+
+    def set_graph(self, enable: bool):
+        """Replay substeps as captured HIP graphs."""
+        check(lib().pbf_set_graph(self._ctx, 1 if enable else 0), self._ctx)
+
+    def set_workgroup_sizes(self, sizes: dict):
+        arr = (ctypes.c_uint32 * len(FAMILIES))(*[int(sizes[f]) for f in FAMILIES])
+        check(lib().pbf_set_workgroup_sizes(self._ctx, arr), self._ctx)
+
+    def set_friends_mode(self, staged: bool):
+        """Friends build variant: LDS-staged neighbourhood or global reads."""
+        check(lib().pbf_set_friends_mode(self._ctx, 1 if staged else 0), self._ctx)
+
+    def autotune(self, steps: int = 3) -> dict:
+        """Times every workgroup size per kernel family; keeps the fastest."""
+        arr = (ctypes.c_uint32 * len(FAMILIES))()
+        check(lib().pbf_autotune(self._ctx, steps, arr), self._ctx)
+        return dict(zip(FAMILIES, (int(a) for a in arr)))
+
+    def stats(self) -> dict:
+        st = PbfStats()
+        check(lib().pbf_stats(self._ctx, ctypes.byref(st)), self._ctx)
+        out = {name: getattr(st, name) for name, _ in PbfStats._fields_ if name not in ("workgroup_size", "reserved")}
+        out["workgroup_size"] = dict(zip(FAMILIES, (int(a) for a in st.workgroup_size)))
+        return out
+
     def error_flags(self, reset: bool = False) -> tuple[int, int]:
         f = ctypes.c_uint32()
         d = ctypes.c_uint64()
@@ -141,6 +172,45 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+class BatchSimulation(Simulation):
+    """Independent scenes stepped together (pbf_create_batch): each stage of
+    a substep is one launch over every scene.  Arrays are the scenes'
+    particles concatenated in scene order; `offsets` gives the ranges."""
+
+    def __init__(self, params_list, options_list=None):
+        self.params_list = [p.copy() for p in params_list]
+        self.options_list = list(options_list) if options_list is not None else [default_options()
+                                                                                   for _ in params_list]
+        self.params = self.params_list[0].copy()
+        self.params.particleCount = sum(int(p.particleCount) for p in self.params_list)
+        self.options = self.options_list[0]
+        self._ctx = None
+        self.bPauseSim = False
+        self.bReadFriendsList = False
+        self.fWavePos = 0.0
+
+    def _ensure(self):
+        if self._ctx is None:
+            S = len(self.params_list)
+            P = (PbfParams * S)(*self.params_list)
+            O = (PbfOptions * S)(*self.options_list)
+            ctx = ctypes.c_void_p()
+            rc = lib().pbf_create_batch(P, O, S, ctypes.byref(ctx))
+            if rc != 0:
+                raise PbfError(rc, lib().pbf_last_error(None).decode())
+            self._ctx = ctx
+
+    def _total_cells(self) -> int:
+        return sum(int(p.gridBufSize) for p in self.params_list)
+
+    @property
+    def offsets(self) -> np.ndarray:
+        self._ensure()
+        off = np.zeros(len(self.params_list) + 1, np.uint32)
+        check(lib().pbf_scene_offsets(self._ctx, off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))), self._ctx)
+        return off
 
 
 def load_parameters(text: str) -> tuple[PbfParams, int]:

@@ -5,6 +5,7 @@ import ctypes
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 import pbf_amd
@@ -114,3 +115,34 @@ def test_scene_generator_is_deterministic():
     assert (a == b).all() and not (a == c).all()
     d = a[:, :3] - ((pbf_amd.dam_break_block(8, 4, 2, 0.1, seed=3, jitter=0.0))[:, :3])
     assert abs(d).max() <= 0.01 * 0.1 + 1e-7
+
+
+@pytest.mark.parametrize("h", [0.1, 0.05, 0.125, 2.0, 0.0731])
+def test_friend_circle_thresholds_match_formula(h):
+    """The kernels' r^2 thresholds reproduce min(int(a^3*5), 4),
+    a = max(0, (sqrt(r^2) - 0.3h)/(h - 0.3h)) (build_friends_list.cl:75-76)
+    exactly, including the binary32 values next to every threshold."""
+    t = (ctypes.c_float * 4)()
+    assert pbf_amd.lib().pbf_friend_circle_thresholds(h, t) == 0
+    t = np.array(list(t), np.float32)
+    hf = np.float32(h)
+    h2 = hf * hf
+    min_r = np.float32(0.3) * hf
+    span = hf - min_r
+
+    def circle(r2):
+        a = (np.sqrt(r2) - min_r) / span
+        a = np.maximum(np.float32(0.0), a)
+        c = (a * a * a * np.float32(5.0)).astype(np.int32)
+        return np.minimum(c, 4)
+
+    rng = np.random.default_rng(0)
+    r2 = (rng.random(200000) * float(h2)).astype(np.float32)
+    near = []
+    for v in t[np.isfinite(t)]:
+        bits = int(np.array([v], np.float32).view(np.uint32)[0])
+        near += [np.array([bits + d], np.uint32).view(np.float32)[0] for d in range(-3, 4)]
+    r2 = np.concatenate([r2, np.array(near, np.float32), np.array([0.0], np.float32)])
+    r2 = r2[r2 < h2]
+    got = (r2[:, None] >= t[None, :]).sum(axis=1)
+    assert np.array_equal(got, circle(r2))

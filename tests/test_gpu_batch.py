@@ -1,0 +1,74 @@
+"""Batched scenes (pbf_create_batch): every scene of the batch must equal
+the oracle run of that scene alone, bit for bit."""
+import numpy as np
+import pytest
+
+import pbf_amd
+import pbf_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def scene_slices(structure, offsets, cell_bases, s, gbs):
+    a, b = int(offsets[s]), int(offsets[s + 1])
+    n_total = int(offsets[-1])
+    n = b - a
+    keys = structure["keys"][a:b].astype(np.int64) - cell_bases[s]
+    perm = structure["perm"][a:b].astype(np.int64) - a
+    cells = structure["cells"][2 * cell_bases[s]: 2 * (cell_bases[s] + gbs)].astype(np.int64)
+    cells = np.where(cells == 0xFFFFFFFF, 0xFFFFFFFF, cells - a)
+    fr = structure["friends"]
+    counts = fr[: 5 * n_total].reshape(5, n_total)[:, a:b]
+    idx = fr[5 * n_total:].reshape(5, 50, n_total)[:, :, a:b].astype(np.int64)
+    mask = np.arange(50)[None, :, None] < counts[:, None, :]
+    idx = np.where(mask, idx - a, 0)
+    return keys, perm, cells, counts, idx
+
+
+def check_batch(scenes, steps, graph=False):
+    sim = pbf_amd.BatchSimulation([s.params for s in scenes], [s.options for s in scenes])
+    sim.set_state(np.concatenate([s.x for s in scenes]), np.concatenate([s.v for s in scenes]))
+    if graph:
+        sim.set_graph(True)
+    sim.run(steps)
+    got = sim.get_state()
+    st = sim.get_structure()
+    off = sim.offsets
+    cell_bases = np.concatenate([[0], np.cumsum([int(s.params.gridBufSize) for s in scenes])])
+    for s, scene in enumerate(scenes):
+        ref = pbf_oracle.OracleSim(pbf_oracle.cfg_from_params(scene.params, scene.options))
+        ref.set_state(scene.x, scene.v)
+        for _ in range(steps):
+            ref.step()
+        want = ref.get_state()
+        rs = ref.get_structure()
+        n = scene.n
+        keys, perm, cells, counts, idx = scene_slices(st, off, cell_bases, s, int(scene.params.gridBufSize))
+        assert np.array_equal(keys, rs["keys"]), s
+        assert np.array_equal(perm, rs["perm"]), s
+        assert np.array_equal(cells, rs["cells"].astype(np.int64)), s
+        rf = rs["friends"]
+        assert np.array_equal(counts, rf[: 5 * n].reshape(5, n)), s
+        assert np.array_equal(idx, rf[5 * n:].reshape(5, 50, n).astype(np.int64)), s
+        a, b = int(off[s]), int(off[s + 1])
+        for k in ("x", "v", "xs", "omega"):
+            assert np.array_equal(got[k][a:b].view(np.uint32), want[k].view(np.uint32)), (s, k)
+        ref.close()
+    sim.close()
+
+
+def test_batch_matches_single_scenes():
+    scenes = pbf_amd.ensemble(6, seed=1, n_min=500, n_max=6000)
+    check_batch(scenes, 3)
+
+
+def test_batch_graph_replay():
+    scenes = pbf_amd.ensemble(5, seed=2, n_min=800, n_max=5000)
+    check_batch(scenes, 4, graph=True)
+
+
+def test_batch_per_scene_gravity():
+    scenes = pbf_amd.ensemble(3, seed=3, n_min=1000, n_max=3000)
+    for s, g in zip(scenes, ((0.0, -9.8, 0.0), (3.0, -9.0, 1.5), (-2.0, -7.0, -6.2))):
+        s.options = pbf_amd.default_options(gravity=g)
+    check_batch(scenes, 2)

@@ -217,3 +217,108 @@ def test_profiling_trackers():
                  "computeScaling_0", "computeDelta_3", "applyViscosity", "applyVorticity"):
         assert name in t and t[name]["calls"] == 3 and t[name]["last_ms"] > 0.0, name
     sim.close()
+
+
+def test_set_state_twice_resets_cells():
+    """A second set_state on the same context starts from clean cells."""
+    scene = pbf_amd.config("dam4k", seed=12)
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 3)
+    other = pbf_amd.config("dam4k", seed=13)
+    sim.set_state(other.x, other.v)
+    ref2 = pbf_oracle.OracleSim(pbf_oracle.cfg_from_params(other.params, other.options))
+    ref2.set_state(other.x, other.v)
+    advance(sim, ref2, 2)
+    check_structure(sim, ref2)
+    check_state(sim, ref2, other)
+    sim.close()
+
+
+def test_graph_replay_matches_eager():
+    """Substeps replayed from captured HIP graphs equal eager launches,
+    checked on every replay with eager host work in between."""
+    scene = pbf_amd.config("dam16k", seed=6)
+    eager = pbf_amd.Simulation(scene.params, scene.options)
+    eager.set_state(scene.x, scene.v)
+    graph = pbf_amd.Simulation(scene.params, scene.options)
+    graph.set_state(scene.x, scene.v)
+    graph.set_graph(True)
+    for s in range(6):
+        eager.Step()
+        graph.Step()
+        a, b = eager.get_state(), graph.get_state()
+        for k in ("x", "v", "xs", "omega"):
+            assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), (s, k)
+    # pause variants are captured as their own graphs
+    eager.bPauseSim = graph.bPauseSim = True
+    eager.Step(); graph.Step()
+    a, b = eager.get_state(), graph.get_state()
+    assert np.array_equal(a["x"].view(np.uint32), b["x"].view(np.uint32))
+    eager.close()
+    graph.close()
+
+
+def test_workgroup_sizes_do_not_change_results():
+    scene = pbf_amd.config("dam16k", seed=7)
+    outs = []
+    for size in (64, 128, 256, 512):
+        sim = pbf_amd.Simulation(scene.params, scene.options)
+        sim.set_state(scene.x, scene.v)
+        sim.set_workgroup_sizes({f: size for f in pbf_amd._lib.FAMILIES})
+        sim.run(4)
+        outs.append(sim.get_state())
+        sim.close()
+    for o in outs[1:]:
+        for k in ("x", "v", "xs", "omega"):
+            assert np.array_equal(outs[0][k].view(np.uint32), o[k].view(np.uint32)), k
+
+
+def test_autotune_restores_state_and_picks_sizes():
+    scene = pbf_amd.config("dam16k", seed=8)
+    ref_sim = pbf_amd.Simulation(scene.params, scene.options)
+    ref_sim.set_state(scene.x, scene.v)
+    ref_sim.run(3)
+    want = ref_sim.get_state()
+    ref_sim.close()
+    sim = pbf_amd.Simulation(scene.params, scene.options)
+    sim.set_state(scene.x, scene.v)
+    chosen = sim.autotune(steps=2)
+    assert set(chosen) == set(pbf_amd._lib.FAMILIES)
+    assert all(v in (64, 128, 256, 512) for v in chosen.values())
+    assert sim.stats()["workgroup_size"] == chosen
+    sim.run(3)
+    got = sim.get_state()
+    for k in ("x", "v", "xs"):
+        assert np.array_equal(got[k].view(np.uint32), want[k].view(np.uint32)), k
+    sim.close()
+
+
+def test_sparse_tiles_fall_back_to_global_scan():
+    """Tiles whose neighbourhood does not fit in LDS scan global memory and
+    give the same friends lists."""
+    rng = np.random.default_rng(3)
+    n = 6000
+    x = np.zeros((n, 4), np.float32)
+    x[:, :3] = rng.random((n, 3)).astype(np.float32) * np.float32(6.0)
+    p = pbf_amd.physical_params(n, 0.1, (0, 0, 0), (6.0, 6.0, 6.0))
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(), x, np.zeros_like(x))
+    sim, ref = make_pair(scene)
+    sim.set_friends_mode(True)
+    advance(sim, ref, 2)
+    assert sim.stats()["unstaged_tiles"] > 0
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()
+
+
+@pytest.mark.parametrize("size", [64, 128, 256, 512])
+def test_staged_friends_build_matches(size):
+    """The LDS-staged friends build gives the oracle's lists at every tile size."""
+    scene = pbf_amd.config("dam16k", seed=14)
+    sim, ref = make_pair(scene)
+    sim.set_friends_mode(True)
+    sim.set_workgroup_sizes({f: size for f in pbf_amd._lib.FAMILIES})
+    advance(sim, ref, 3)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()

@@ -289,28 +289,38 @@ void pbfo_pack(uint32_t n, const float *src4, const float *w, float *dst4)
  * plane whose previous position was within the force zone is moved onto the
  * offset plane.  Planes are applied in the order x-lo, x-hi, y-lo, y-hi,
  * z-lo, z-hi, as the reference chains its quads (compute_delta.cl:169-182). */
-static inline void wall_lo(float *f, const float *prev, int a, float lo, float edge)
+static inline int wall_lo(float *f, const float *prev, int a, float lo, float edge)
 {
     const float pa = lo + edge;
     const float dp = f[a] - pa;
-    if (dp > 0.0f) return;                       /* dot(n, deltaP) > 0   */
+    if (dp > 0.0f) return 0;                     /* dot(n, deltaP) > 0   */
     const float nv = prev[a] - f[a];             /* dot(n, prev - next)  */
-    if (fabsf(dp) > edge + nv) return;           /* outside force zone   */
+    if (fabsf(dp) > edge + nv) return 0;         /* outside force zone   */
     f[a] = pa;
+    return 1;
 }
 
-static inline void wall_hi(float *f, const float *prev, int a, float hi, float edge)
+static inline int wall_hi(float *f, const float *prev, int a, float hi, float edge)
 {
     const float pa = hi - edge;
     const float dp = f[a] - pa;
-    if (dp < 0.0f) return;
+    if (dp < 0.0f) return 0;
     const float nv = f[a] - prev[a];
-    if (fabsf(dp) > edge + nv) return;
+    if (fabsf(dp) > edge + nv) return 0;
     f[a] = pa;
+    return 1;
 }
 
-/* compute_delta.cl:44-186 (delta.w is left untouched, as in the reference). */
 void pbfo_delta(const pbfo_cfg *c, const float *x, const float *xs, const uint32_t *fr, float *delta)
+{
+    pbfo_delta_hits(c, x, xs, fr, delta, NULL);
+}
+
+/* compute_delta.cl:44-186 (delta.w is left untouched, as in the reference).
+ * hits (optional): per particle, which planes moved it (bit 2a: axis a low,
+ * bit 2a+1: axis a high). */
+void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const uint32_t *fr, float *delta,
+                     uint8_t *hits)
 {
     const uint32_t n = c->n;
     const float h = c->h, h2 = h * h, rd = c->rest_density, K = c->st_k;
@@ -357,10 +367,14 @@ void pbfo_delta(const pbfo_cfg *c, const float *x, const float *xs, const uint32
             /* compute_delta.cl:166 without its scene-scale "1.0 +" term */
             const float edge = (3.0f + ns[0] + ns[1] + ns[2]) * h * 0.03f;
             const float *prev = F4(x, i);
+            int hb = 0;
             for (int a = 0; a < 3; ++a) {
-                wall_lo(fut, prev, a, c->box_lo[a], edge);
-                wall_hi(fut, prev, a, c->box_hi[a], edge);
+                hb |= wall_lo(fut, prev, a, c->box_lo[a], edge) << (2 * a);
+                hb |= wall_hi(fut, prev, a, c->box_hi[a], edge) << (2 * a + 1);
             }
+            if (hits) hits[i] = (uint8_t)hb;
+        } else if (hits) {
+            hits[i] = 0;
         }
         float *o = F4(delta, i);
         for (int k = 0; k < 3; ++k) o[k] = fut[k] - pi[k];
@@ -496,6 +510,7 @@ struct pbfo_sim {
     pbfo_cfg c;
     float *x, *v, *xs, *xs2, *rho, *lambda, *omega, *delta, *tmp4;
     uint32_t *keys, *keys_sorted, *perm, *cells, *friends;
+    uint8_t *wall_hits;  /* iters x n, written by each step when set */
     int sorted_once;
 };
 
@@ -560,7 +575,8 @@ void pbfo_step(pbfo_sim *s, uint32_t pause)
     for (uint32_t it = 0; it < c->iters; ++it) {
         pbfo_scaling(c, s->xs, s->friends, s->rho, s->lambda);
         pbfo_pack(n, s->xs, s->lambda, s->xs2);
-        pbfo_delta(c, s->x, s->xs2, s->friends, s->delta);
+        pbfo_delta_hits(c, s->x, s->xs2, s->friends, s->delta,
+                        s->wall_hits ? s->wall_hits + (size_t)it * n : NULL);
         pbfo_update_predicted(n, s->xs2, s->delta, s->xs);
     }
     pbfo_pack(n, s->xs, s->rho, s->xs2);
@@ -570,6 +586,8 @@ void pbfo_step(pbfo_sim *s, uint32_t pause)
     memcpy(s->v, s->tmp4, 16 * (size_t)n);
     pbfo_vorticity(c, s->xs, s->v, s->omega, s->friends);
 }
+
+void pbfo_record_walls(pbfo_sim *s, uint8_t *hits) { s->wall_hits = hits; }
 
 void *pbfo_buffer(pbfo_sim *s, int which)
 {

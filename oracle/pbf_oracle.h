@@ -73,6 +73,8 @@ void pbfo_friends(const pbfo_cfg *c, const float *xs, const uint32_t *cells, uin
 void pbfo_scaling(const pbfo_cfg *c, const float *xs, const uint32_t *friends, float *rho, float *lambda);
 void pbfo_pack(uint32_t n, const float *src4, const float *w, float *dst4);
 void pbfo_delta(const pbfo_cfg *c, const float *x, const float *xs, const uint32_t *friends, float *delta);
+void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const uint32_t *friends, float *delta,
+                     uint8_t *hits);
 void pbfo_update_predicted(uint32_t n, const float *xs, const float *delta, float *xs_out);
 void pbfo_update_velocities(const pbfo_cfg *c, float *x, const float *xs, float *v);
 void pbfo_viscosity(const pbfo_cfg *c, const float *xs, const float *v_in, const uint32_t *friends,
@@ -92,6 +94,9 @@ void pbfo_step(pbfo_sim *s, uint32_t pause);
 /* which: 0 x, 1 v, 2 xs (w = rho), 3 rho, 4 lambda, 5 omega, 6 keys_sorted,
  * 7 perm, 8 cells, 9 friends, 10 keys (unsorted, this step) */
 void *pbfo_buffer(pbfo_sim *s, int which);
+/* Each later step writes, per solver iteration, which wall planes moved each
+ * particle (iters x n bytes, bits as pbfo_delta_hits); NULL stops it. */
+void pbfo_record_walls(pbfo_sim *s, uint8_t *hits);
 int pbfo_threads(void);
 
 #ifdef __cplusplus

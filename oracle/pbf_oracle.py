@@ -80,6 +80,12 @@ def lib() -> ctypes.CDLL:
             "pbfo_step": (None, [vp, ctypes.c_uint32]),
             "pbfo_buffer": (vp, [vp, ctypes.c_int]),
             "pbfo_threads": (ctypes.c_int, []),
+            "pbfo_record_walls": (None, [vp, vp]),
+            "pbfa_rollout_f64": (None, [cp, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "pbfa_jvp_f64": (None, [cp, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "pbfa_jvp_f32": (None, [cp, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "pbfa_tangent_fd": (None, [cp, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                       ctypes.c_double, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -176,6 +182,113 @@ class OracleSim:
             self.close()
         except Exception:
             pass
+
+
+def _opt(a, dtype):
+    return None if a is None else ptr(np.ascontiguousarray(a, dtype))
+
+
+class FrozenRollout:
+    """A rollout of `steps` substeps with its integer structure frozen.
+
+    The binary32 oracle runs the substeps from (x0, v0) and records, per
+    substep, the sort permutation, the friend lists (reference layout) and the
+    wall planes that moved each particle in each iteration.  With that
+    structure held fixed the rollout is a smooth function of the initial
+    state and of coef = (viscosity, vorticity, g0, g1, g2), which
+    pbf_adjoint_ref.cpp evaluates in FP64 (central differences), as FP64 dual
+    numbers (JVP) and as binary32 values with FP64 tangents (the derivative of
+    the exact binary32 trajectory).  Arrays are (n, 3) in particle-identity
+    order."""
+
+    def __init__(self, cfg: Cfg, x0, v0, steps: int):
+        self.cfg = cfg
+        self.n = n = int(cfg.n)
+        self.steps = steps
+        iters = int(cfg.iters)
+        x4 = np.zeros((n, 4), np.float32)
+        v4 = np.zeros((n, 4), np.float32)
+        x4[:, :3] = np.asarray(x0, np.float32)[:, :3]
+        v4[:, :3] = np.asarray(v0, np.float32)[:, :3]
+        self.x0 = x4[:, :3].copy()
+        self.v0 = v4[:, :3].copy()
+        self.coef0 = np.array([cfg.viscosity, cfg.vorticity, cfg.gravity[0], cfg.gravity[1], cfg.gravity[2]],
+                              np.float32)
+        sim = OracleSim(cfg)
+        sim.set_state(x4, v4)
+        hits = np.zeros((iters, n), np.uint8)
+        lib().pbfo_record_walls(sim._s, ptr(hits))
+        self.perm = np.zeros((steps, n), np.uint32)
+        self.friends = np.zeros((steps, n * CIRCLES * (1 + PER_CIRCLE)), np.uint32)
+        self.hits = np.zeros((steps, iters, n), np.uint8)
+        ids = np.arange(n)
+        for s in range(steps):
+            sim.step()
+            st = sim.get_structure()
+            self.perm[s] = st["perm"]
+            self.friends[s] = st["friends"]
+            self.hits[s] = hits
+            ids = ids[st["perm"]]
+        lib().pbfo_record_walls(sim._s, None)
+        state = sim.get_state()
+        sim.close()
+        # the binary32 oracle's final state in identity order
+        self.xT32 = np.zeros((n, 3), np.float32)
+        self.vT32 = np.zeros((n, 3), np.float32)
+        self.xT32[ids] = state["x"][:, :3]
+        self.vT32[ids] = state["v"][:, :3]
+        self.ids = ids
+
+    def _structure(self):
+        return (ctypes.byref(self.cfg), self.steps, ptr(self.perm), ptr(self.friends), ptr(self.hits))
+
+    def rollout64(self, x0=None, v0=None, coef=None):
+        n = self.n
+        x0 = np.ascontiguousarray(self.x0 if x0 is None else x0, np.float64)
+        v0 = np.ascontiguousarray(self.v0 if v0 is None else v0, np.float64)
+        coef = np.ascontiguousarray(self.coef0 if coef is None else coef, np.float64)
+        xT = np.zeros((n, 3))
+        vT = np.zeros((n, 3))
+        lib().pbfa_rollout_f64(*self._structure(), ptr(x0), ptr(v0), ptr(coef), ptr(xT), ptr(vT))
+        return xT, vT
+
+    def jvp64(self, dx0=None, dv0=None, dcoef=None, x0=None, v0=None, coef=None):
+        n = self.n
+        x0 = np.ascontiguousarray(self.x0 if x0 is None else x0, np.float64)
+        v0 = np.ascontiguousarray(self.v0 if v0 is None else v0, np.float64)
+        coef = np.ascontiguousarray(self.coef0 if coef is None else coef, np.float64)
+        out = [np.zeros((n, 3)) for _ in range(4)]
+        lib().pbfa_jvp_f64(*self._structure(), ptr(x0), ptr(v0), ptr(coef), _opt(dx0, np.float64),
+                           _opt(dv0, np.float64), _opt(dcoef, np.float64), *[ptr(o) for o in out])
+        return tuple(out)
+
+    def jvp32(self, dx0=None, dv0=None, dcoef=None):
+        n = self.n
+        x0 = np.ascontiguousarray(self.x0, np.float32)
+        v0 = np.ascontiguousarray(self.v0, np.float32)
+        coef = np.ascontiguousarray(self.coef0, np.float32)
+        xT = np.zeros((n, 3), np.float32)
+        vT = np.zeros((n, 3), np.float32)
+        dxT = np.zeros((n, 3))
+        dvT = np.zeros((n, 3))
+        lib().pbfa_jvp_f32(*self._structure(), ptr(x0), ptr(v0), ptr(coef), _opt(dx0, np.float64),
+                           _opt(dv0, np.float64), _opt(dcoef, np.float64), ptr(xT), ptr(vT), ptr(dxT), ptr(dvT))
+        return xT, vT, dxT, dvT
+
+
+    def tangent_fd(self, dx0=None, dv0=None, dcoef=None, delta: float = 1e-6):
+        """FP64 central differences of each substep, chained along the
+        binary32 trajectory (perturbations of `delta` in the largest
+        component)."""
+        n = self.n
+        x0 = np.ascontiguousarray(self.x0, np.float32)
+        v0 = np.ascontiguousarray(self.v0, np.float32)
+        coef = np.ascontiguousarray(self.coef0, np.float32)
+        dxT = np.zeros((n, 3))
+        dvT = np.zeros((n, 3))
+        lib().pbfa_tangent_fd(*self._structure(), ptr(x0), ptr(v0), ptr(coef), _opt(dx0, np.float64),
+                              _opt(dv0, np.float64), _opt(dcoef, np.float64), delta, ptr(dxT), ptr(dvT))
+        return dxT, dvT
 
 
 def kernel_factors(h: float) -> tuple[float, float]:

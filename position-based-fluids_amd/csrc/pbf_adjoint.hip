@@ -1,0 +1,998 @@
+// pbf_adjoint.hip -- reverse-mode adjoint of the substep (include/pbf/pbf_adjoint.h).
+//
+// Forward of one substep s, in the cell-sorted order of s (perm_s), with the
+// friend lists L05/L06 (the 0.5 / 0.6 circle prefixes) frozen:
+//   P0      = X + dt (V + dt g)                      (predictPositions, gathered)
+//   k < K:  lambda_k = Lambda(P_k)                   (computeScaling)
+//           P_k+1    = Wall(P_k + Dp(P_k, lambda_k)) (computeDelta + walls)
+//   Vu = (P_K - Xs) / dt                             (updateVelocities)
+//   Vx, Om = Xsph(P_K, Vu; c)                        (applyViscosity, Jacobi)
+//   V' = Vx + dt eps N(P_K, Om) x Om                 (applyVorticity)
+//   X' = P_K
+// The backward walks it in reverse with one kernel per sweep.  A pair term
+// f(p_i - p_j) of particle i's sum gives gradient to both i and j; instead of
+// scattering to j with atomics, every kernel gathers for its particle m both
+// its own pairs (its list) and the pairs in which m is the friend (the
+// transposed list T_m: the particles i whose processed prefix holds m).  Each
+// thread then owns its outputs and the sums are deterministic.
+//
+// Transposed lists.  Without friend drops (the 50 cap) and without duplicate
+// bucket visits, i lists m in circle c iff m lists i in circle c: r^2 is
+// computed from an exactly antisymmetric difference and the 27-cell
+// neighbourhoods are symmetric.  So T_m = {i in m's full list, circle c :
+// c < circles processed by i}, built when the substep is recorded; drops and
+// duplicate buckets are detected and make the backward fail rather than
+// differentiate a different function.
+//
+// Per-pair derivatives, r = p_i - p_j, q = |r|, d = h^2 - q^2:
+//   psi(r) = r (h-q)^2 / q            (the spiky direction without SPIKY)
+//   J(r) u = w u + (w'/q)(r.u) r,  w = (h-q)^2/q, w' = -(h-q)(h+q)/q^2
+//   grad d^3 = -6 d^2 r
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <new>
+#include <vector>
+
+#include "../../include/pbf/pbf_adjoint.h"
+#include "pbf_internal.h"
+
+namespace pbf {
+namespace {
+
+constexpr int kAdjThreads = 256;   // workgroup of every adjoint kernel (block table bt[2])
+constexpr int kParts = 8;          // per-tile partial sums: [0] viscosity, [1] vorticity, [2..4] gravity
+constexpr uint32_t kDupFlag = 1u;  // a particle's 27-cell neighbourhood visits one bucket twice
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(const float4& a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3& operator+=(V3& a, V3 b) { a = a + b; return a; }
+__device__ __forceinline__ V3& operator-=(V3& a, V3 b) { a = a - b; return a; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float4 f4(V3 a) { return make_float4(a.x, a.y, a.z, 0.0f); }
+
+// Geometry of an in-range pair.
+struct Pair {
+    V3 r;
+    float r2, q, hr, d;
+};
+
+__device__ __forceinline__ bool pair_of(const float4& a, const float4& b, const SceneConsts& c, Pair& g) {
+    g.r = {a.x - b.x, a.y - b.y, a.z - b.z};
+    g.r2 = dot(g.r, g.r);
+    if (!(g.r2 < c.h2)) return false;
+    g.q = sqrtf(g.r2);
+    g.hr = c.h - g.q;
+    g.d = c.h2 - g.r2;
+    return true;
+}
+
+__device__ __forceinline__ V3 psi(const Pair& g) { return g.r * (g.hr * g.hr / g.q); }
+
+__device__ __forceinline__ V3 jpsi(const Pair& g, V3 u, const SceneConsts& c) {
+    const float w = g.hr * g.hr / g.q;
+    const float wq = -(g.hr * (c.h + g.q)) / (g.r2 * g.q);   // w'/q
+    return u * w + g.r * (wq * dot(g.r, u));
+}
+
+// Transposed entries: count of the 0.6 list in the low half, of its 0.5
+// prefix in the high half.
+__device__ __forceinline__ int t06(uint32_t tc) { return (int)(tc & 0xFFFFu); }
+__device__ __forceinline__ int t05(uint32_t tc) { return (int)(tc >> 16); }
+
+// Sum over the workgroup of NV values per thread (fixed order), written by
+// thread 0 to out[0..NV).
+template <int NV>
+__device__ __forceinline__ void block_sum(float (&v)[NV], float* out) {
+    __shared__ float s[kAdjThreads / 64][NV];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) s[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            float a = 0.0f;
+            for (int q = 0; q < kAdjThreads / 64; ++q) a += s[q][k];
+            out[k] = a;
+        }
+    }
+}
+
+// Transpose of the walls (compute_delta.cl:1-42 as applied in apply_walls):
+// a coordinate a plane moved becomes lo + edge(F) or hi - edge(F), with
+// edge(F) = (3 + sum_c sin(5 F_c / h)) 0.03 h; the others pass through.
+__device__ __forceinline__ V3 wall_vjp(const float4& fut, V3 g, const SceneConsts& c) {
+    if (!c.walls) return g;
+    const uint32_t hits = __float_as_uint(fut.w);
+    if (!hits) return g;
+    const float sx = (hits & 2u) ? -1.0f : ((hits & 1u) ? 1.0f : 0.0f);
+    const float sy = (hits & 8u) ? -1.0f : ((hits & 4u) ? 1.0f : 0.0f);
+    const float sz = (hits & 32u) ? -1.0f : ((hits & 16u) ? 1.0f : 0.0f);
+    V3 o = {sx != 0.0f ? 0.0f : g.x, sy != 0.0f ? 0.0f : g.y, sz != 0.0f ? 0.0f : g.z};
+    const float e = sx * g.x + sy * g.y + sz * g.z;
+    const float k = 0.15f * e;   // d edge / dF_c = 0.03 h * 5/h * cos(5 F_c / h)
+    o.x += k * cosf(fut.x * 5.0f / c.h);
+    o.y += k * cosf(fut.y * 5.0f / c.h);
+    o.z += k * cosf(fut.z * 5.0f / c.h);
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// Recording
+// ---------------------------------------------------------------------------
+
+// T_m from m's full friend list (all circles, particle-major segments): the
+// friends i whose processed prefix contains m's circle, 0.5 members first.
+__global__ __launch_bounds__(kAdjThreads) void k_transpose(const SceneConsts* __restrict__ scenes,
+                                                          const uint4* __restrict__ blocks,
+                                                          const float4* __restrict__ xs,
+                                                          const uint32_t* __restrict__ cnt,
+                                                          const uint32_t* __restrict__ nb_list,
+                                                          const uint2* __restrict__ cells,
+                                                          uint32_t* __restrict__ tcnt, uint4* __restrict__ tquads,
+                                                          uint32_t* flags) {
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t m = be.y + threadIdx.x;
+    if (m >= be.z) return;
+    const uint32_t pm = cnt[m];
+    uint32_t acc[4] = {0u, 0u, 0u, 0u};
+    int na = 0, qo = 0, n05 = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int q = 0; q < kCircles; ++q) {
+            const int cq = circle_count(pm, q);
+            const uint32_t* seg = nb_list + friend_seg(m, q);
+            for (int f = 0; f < cq; ++f) {
+                const uint32_t i = seg[f];
+                const uint32_t pi = cnt[i];
+                int dummy;
+                const int l05 = circles_processed<1, 2>(pi, &dummy);
+                const int l06 = circles_processed<3, 5>(pi, &dummy);
+                const bool take = pass == 0 ? (q < l05) : (q >= l05 && q < l06);
+                if (take) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) acc[u] = (u == na) ? i : acc[u];
+                    if (++na == 4) {
+                        tquads[sweep_quad(m, qo++)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
+                        na = 0;
+                    }
+                }
+            }
+        }
+        if (pass == 0) n05 = 4 * qo + na;
+    }
+    if (na) tquads[sweep_quad(m, qo)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
+    tcnt[m] = (uint32_t)(4 * qo + na) | ((uint32_t)n05 << 16);
+
+    // Symmetry needs every bucket visited once.  With gbs = 2^b, b >= 6, the
+    // 27 neighbour cells always hash apart (their coordinates differ mod 4).
+    if (!(c.gbs_mask != 0u && c.gbs >= 64u)) {
+        const float4 p = xs[m];
+        const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
+        uint32_t keys[27];
+        int nk = 0;
+        bool dup = false;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const uint32_t k = cell_hash(cx + dx, cy + dy, cz + dz, c);
+                    if (cells[k].x == kEmpty) continue;
+                    for (int t = 0; t < nk; ++t) dup |= keys[t] == k;
+                    keys[nk++] = k;
+                }
+        if (dup) atomicOr(flags, kDupFlag);
+    }
+}
+
+// Snapshot of the dropped-friends counter (err words 2..3).
+__global__ void k_note_drops(const uint32_t* __restrict__ err, unsigned long long* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *out = (unsigned long long)err[2] | ((unsigned long long)err[3] << 32);
+}
+
+__global__ void k_iota(uint32_t* __restrict__ ids, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) ids[i] = i;
+}
+
+// ids after a substep: the identity of the particle at sorted slot i.
+__global__ void k_ids_gather(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ ids_in,
+                             uint32_t* __restrict__ ids_out, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const uint32_t p = perm[i];
+        ids_out[i] = p < n ? ids_in[p] : 0u;
+    }
+}
+
+// Sorted state -> identity order.
+__global__ void k_scatter_state(const uint32_t* __restrict__ ids, const float4* __restrict__ x,
+                                const float4* __restrict__ v, float4* __restrict__ xo, float4* __restrict__ vo,
+                                uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t d = ids[i];
+    if (d >= n) return;
+    if (xo) xo[d] = x[i];
+    if (vo) vo[d] = v[i];
+}
+
+// Identity-order gradients -> sorted order of the last substep (w cleared).
+__global__ void k_gather_grads(const uint32_t* __restrict__ ids, const float4* __restrict__ gx_in,
+                               const float4* __restrict__ gv_in, float4* __restrict__ gx, float4* __restrict__ gv,
+                               uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t d = ids[i];
+    const float4 a = (gx_in && d < n) ? gx_in[d] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 b = (gv_in && d < n) ? gv_in[d] : make_float4(0.f, 0.f, 0.f, 0.f);
+    gx[i] = make_float4(a.x, a.y, a.z, 0.0f);
+    gv[i] = make_float4(b.x, b.y, b.z, 0.0f);
+}
+
+__global__ void k_fill(float* __restrict__ p, size_t count) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) p[i] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Backward kernels, in the order they run for one substep
+// ---------------------------------------------------------------------------
+
+// applyVorticity, own part (apply_vorticity.cl:1-87): recomputes eta and
+// N = eta/|eta| of particle i, gives dL/deps, the direct dL/dOm_i, and the
+// pair coefficient zeta_i = -SPIKY dL/deta_i.
+__global__ __launch_bounds__(kAdjThreads) void k_vort_b1(const SceneConsts* __restrict__ scenes,
+                                                        const uint4* __restrict__ blocks,
+                                                        const float4* __restrict__ P, const float4* __restrict__ om,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const uint4* __restrict__ quads,
+                                                        const float4* __restrict__ gv, float4* __restrict__ z,
+                                                        float4* __restrict__ gom, float* __restrict__ part) {
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint4 be = blocks[tile];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t i = be.y + threadIdx.x;
+    float ge[1] = {0.0f};
+    if (i < be.z) {
+        const float4 p = P[i];
+        V3 eta = {0.f, 0.f, 0.f};
+        auto add = [&](uint32_t j) {
+            const float4 pj = P[j];
+            Pair g;
+            if (pair_of(p, pj, c, g) && fabsf(pj.w) > 1e-8f) {
+                const V3 oj = v3(om[j]);
+                eta += psi(g) * sqrtf(dot(oj, oj));
+            }
+        };
+        for_friends<1, 2>(
+            i, cnt[i], quads, [&](const uint4& jj) { add(jj.x); add(jj.y); add(jj.z); add(jj.w); }, add);
+        eta = eta * -c.spiky;
+        const float l = sqrtf(dot(eta, eta));
+        const V3 N = l > 0.0f ? eta * (1.0f / l) : eta;
+        const V3 oi = v3(om[i]);
+        const V3 g = v3(gv[i]);
+        ge[0] = c.dt * dot(cross(N, oi), g);
+        const float u = c.vorticity * c.dt;
+        const V3 gN = cross(oi, g) * u;
+        const V3 gOd = cross(g, N) * u;
+        const V3 geta = l > 0.0f ? (gN - N * dot(N, gN)) * (1.0f / l) : gN;
+        z[i] = f4(geta * -c.spiky);
+        gom[i] = f4(gOd);
+    }
+    block_sum<1>(ge, part + (size_t)tile * kParts + 1);
+}
+
+// applyVorticity, pair part: dL/dP_K and dL/dOm from the pairs of m's list
+// and of its transposed list.
+__global__ __launch_bounds__(kAdjThreads) void k_vort_b2(
+    const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ P,
+    const float4* __restrict__ om, const uint32_t* __restrict__ cnt, const uint4* __restrict__ quads,
+    const uint32_t* __restrict__ tcnt, const uint4* __restrict__ tquads, const float4* __restrict__ z,
+    float4* __restrict__ gom, float4* __restrict__ gx) {
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t m = be.y + threadIdx.x;
+    if (m >= be.z) return;
+    const float4 p = P[m];
+    const V3 zm = v3(z[m]);
+    V3 gP = {0.f, 0.f, 0.f};
+    auto own = [&](uint32_t j) {
+        const float4 pj = P[j];
+        Pair g;
+        if (pair_of(p, pj, c, g) && fabsf(pj.w) > 1e-8f) {
+            const V3 oj = v3(om[j]);
+            gP += jpsi(g, zm, c) * sqrtf(dot(oj, oj));
+        }
+    };
+    for_friends<1, 2>(
+        m, cnt[m], quads, [&](const uint4& jj) { own(jj.x); own(jj.y); own(jj.z); own(jj.w); }, own);
+    if (fabsf(p.w) > 1e-8f) {
+        const V3 omm = v3(om[m]);
+        const float wl = sqrtf(dot(omm, omm));
+        float gs = 0.0f;
+        auto tr = [&](uint32_t i) {
+            Pair g;
+            if (pair_of(P[i], p, c, g)) {
+                const V3 zi = v3(z[i]);
+                gP -= jpsi(g, zi, c) * wl;
+                gs += dot(psi(g), zi);
+            }
+        };
+        for_quads(m, t05(tcnt[m]), tquads, [&](const uint4& ii) { tr(ii.x); tr(ii.y); tr(ii.z); tr(ii.w); }, tr);
+        if (wl > 0.0f) gom[m] = f4(v3(gom[m]) + omm * (gs / wl));
+    }
+    gx[m] = f4(v3(gx[m]) + gP);
+}
+
+// applyViscosity (apply_viscosity.cl:1-87, Jacobi) and updateVelocities
+// (update_velocities.cl:13): dL/dVu, dL/dP_K, dL/dc; then the walls of the
+// last iteration: gx <- dL/dF_{K-1}, z <- kappa = -SPIKY/rho0 dL/dF.
+__global__ __launch_bounds__(kAdjThreads) void k_xsph_b(
+    const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ P,
+    const float4* __restrict__ vu, const uint32_t* __restrict__ cnt, const uint4* __restrict__ quads,
+    const uint32_t* __restrict__ tcnt, const uint4* __restrict__ tquads, const float4* __restrict__ gv,
+    const float4* __restrict__ gom, float4* __restrict__ gx, float4* __restrict__ gxs, float4* __restrict__ z,
+    const float4* __restrict__ fut, float* __restrict__ part) {
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint4 be = blocks[tile];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t m = be.y + threadIdx.x;
+    float gc[1] = {0.0f};
+    if (m < be.z) {
+        const float4 p = P[m];
+        const V3 vm = v3(vu[m]);
+        const V3 gvm = v3(gv[m]);
+        const float cp6 = c.viscosity * c.poly6;
+        const V3 am = gvm * cp6;
+        const V3 gam = v3(gom[m]) * -c.spiky;
+        V3 S = {0.f, 0.f, 0.f}, gVu = {0.f, 0.f, 0.f}, gP = {0.f, 0.f, 0.f};
+        auto own = [&](uint32_t j) {
+            const float4 pj = P[j];
+            Pair g;
+            if (pair_of(p, pj, c, g) && fabsf(pj.w) > 1e-8f) {
+                const V3 u = v3(vu[j]) - vm;
+                const float W = g.d * g.d * g.d;
+                const V3 ps = psi(g);
+                S += u * W;
+                gVu -= am * W + cross(ps, gam);
+                gP += g.r * (-6.0f * g.d * g.d * dot(u, am)) + jpsi(g, cross(gam, u), c);
+            }
+        };
+        for_friends<1, 2>(
+            m, cnt[m], quads, [&](const uint4& jj) { own(jj.x); own(jj.y); own(jj.z); own(jj.w); }, own);
+        gc[0] = dot(S * c.poly6, gvm);
+        if (fabsf(p.w) > 1e-8f) {
+            auto tr = [&](uint32_t i) {
+                Pair g;
+                if (pair_of(P[i], p, c, g)) {
+                    const V3 u = vm - v3(vu[i]);
+                    const V3 ai = v3(gv[i]) * cp6;
+                    const V3 gai = v3(gom[i]) * -c.spiky;
+                    const float W = g.d * g.d * g.d;
+                    gVu += ai * W + cross(psi(g), gai);
+                    gP -= g.r * (-6.0f * g.d * g.d * dot(u, ai)) + jpsi(g, cross(gai, u), c);
+                }
+            };
+            for_quads(m, t05(tcnt[m]), tquads, [&](const uint4& ii) { tr(ii.x); tr(ii.y); tr(ii.z); tr(ii.w); },
+                      tr);
+        }
+        gVu += gvm;
+        const V3 gPK = v3(gx[m]) + gP + gVu * (1.0f / c.dt);
+        gxs[m] = f4(gVu * (-1.0f / c.dt));
+        const V3 gF = wall_vjp(fut[m], gPK, c);
+        gx[m] = f4(gF);
+        z[m] = f4(gF * (-c.spiky / c.rest_density));
+    }
+    block_sum<1>(gc, part + (size_t)tile * kParts + 0);
+}
+
+// computeDelta (compute_delta.cl:44-186) of iteration k: dL/dlambda_k and
+// the pair part of dL/dP_k; then the pointwise coefficients of
+// computeScaling's adjoint: with lambda = -C/Den,
+//   A = gl POLY6 / (rho0 Den),  B = gl 2 SPIKY C / (rho0^2 Den^2),  H = B G.
+__global__ __launch_bounds__(kAdjThreads) void k_delta_b(
+    const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ P,
+    const float4* __restrict__ aux, const uint32_t* __restrict__ cnt, const uint4* __restrict__ quads,
+    const uint32_t* __restrict__ tcnt, const uint4* __restrict__ tquads, const float4* __restrict__ z,
+    float4* __restrict__ gx, float4* __restrict__ z2, float* __restrict__ zb) {
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t m = be.y + threadIdx.x;
+    if (m >= be.z) return;
+    const float4 p = P[m];
+    const V3 km = v3(z[m]);
+    const float ks = -24.0f * c.st_k / c.poly6_q;
+    float gl = 0.0f;
+    V3 gP = {0.f, 0.f, 0.f};
+    auto own = [&](uint32_t j) {
+        const float4 pj = P[j];
+        Pair g;
+        if (pair_of(p, pj, c, g)) {
+            const V3 ps = psi(g);
+            const float ratio = g.d * g.d * g.d / c.poly6_q;
+            const float r3 = ratio * ratio * ratio;
+            const float coef = p.w + pj.w + c.st_k * r3 * ratio;
+            const float t = dot(ps, km);
+            gl += t;
+            gP += jpsi(g, km, c) * coef + g.r * (t * ks * r3 * g.d * g.d);
+        }
+    };
+    for_friends<1, 2>(
+        m, cnt[m], quads, [&](const uint4& jj) { own(jj.x); own(jj.y); own(jj.z); own(jj.w); }, own);
+    auto tr = [&](uint32_t i) {
+        const float4 pi = P[i];
+        Pair g;
+        if (pair_of(pi, p, c, g)) {
+            const V3 ki = v3(z[i]);
+            const V3 ps = psi(g);
+            const float ratio = g.d * g.d * g.d / c.poly6_q;
+            const float r3 = ratio * ratio * ratio;
+            const float coef = pi.w + p.w + c.st_k * r3 * ratio;
+            const float t = dot(ps, ki);
+            gl += t;
+            gP -= jpsi(g, ki, c) * coef + g.r * (t * ks * r3 * g.d * g.d);
+        }
+    };
+    for_quads(m, t05(tcnt[m]), tquads, [&](const uint4& ii) { tr(ii.x); tr(ii.y); tr(ii.z); tr(ii.w); }, tr);
+    gx[m] = f4(v3(gx[m]) + gP);
+    const float4 a4 = aux[m];
+    const float den = a4.w;
+    const float C = -p.w * den;
+    const float rd = c.rest_density;
+    const float A = gl * c.poly6 / (rd * den);
+    const float B = gl * 2.0f * c.spiky * C / (rd * rd * den * den);
+    z2[m] = make_float4(B * a4.x, B * a4.y, B * a4.z, A);
+    zb[m] = B;
+}
+
+// computeScaling (compute_scaling.cl:1-113) of iteration k: the pair part of
+// dL/dP_k, e(r) = 6A d^2 r + J(r)(B SPIKY psi(r) + H).  Then either the
+// walls of iteration k-1 (gx <- dL/dF_{k-1}, z <- kappa), or for k = 0 the
+// reorder and predictPositions: dL/dX[perm] = dL/dXs + dL/dP0,
+// dL/dV[perm] = dt dL/dP0, dL/dg += dt^2 dL/dP0.
+template <bool FIRST>
+__global__ __launch_bounds__(kAdjThreads) void k_lambda_b(
+    const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ P,
+    const uint32_t* __restrict__ cnt, const uint4* __restrict__ quads, const uint32_t* __restrict__ tcnt,
+    const uint4* __restrict__ tquads, const float4* __restrict__ z2, const float* __restrict__ zb,
+    float4* __restrict__ gx, const float4* __restrict__ fut_prev, float4* __restrict__ z,
+    const uint32_t* __restrict__ perm, const float4* __restrict__ gxs, float4* __restrict__ gx_out,
+    float4* __restrict__ gv_out, float* __restrict__ part, uint32_t n) {
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint4 be = blocks[tile];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t m = be.y + threadIdx.x;
+    float gg[3] = {0.0f, 0.0f, 0.0f};
+    if (m < be.z) {
+        const float4 p = P[m];
+        const float4 qm = z2[m];
+        const V3 Hm = v3(qm);
+        const float Am = qm.w, Bm = zb[m];
+        V3 gP = {0.f, 0.f, 0.f};
+        auto own = [&](uint32_t j) {
+            Pair g;
+            if (pair_of(p, P[j], c, g)) {
+                const V3 ps = psi(g);
+                gP += g.r * (6.0f * Am * g.d * g.d) + jpsi(g, ps * (Bm * c.spiky) + Hm, c);
+            }
+        };
+        for_friends<3, 5>(
+            m, cnt[m], quads, [&](const uint4& jj) { own(jj.x); own(jj.y); own(jj.z); own(jj.w); }, own);
+        auto tr = [&](uint32_t i) {
+            Pair g;
+            if (pair_of(P[i], p, c, g)) {
+                const float4 qi = z2[i];
+                const float Bi = zb[i];
+                const V3 ps = psi(g);
+                gP -= g.r * (6.0f * qi.w * g.d * g.d) + jpsi(g, ps * (Bi * c.spiky) + v3(qi), c);
+            }
+        };
+        for_quads(m, t06(tcnt[m]), tquads, [&](const uint4& ii) { tr(ii.x); tr(ii.y); tr(ii.z); tr(ii.w); }, tr);
+        const V3 gPk = v3(gx[m]) + gP;
+        if (!FIRST) {
+            const V3 gF = wall_vjp(fut_prev[m], gPk, c);
+            gx[m] = f4(gF);
+            z[m] = f4(gF * (-c.spiky / c.rest_density));
+        } else {
+            const uint32_t pp = perm[m];
+            if (pp < n) {
+                gx_out[pp] = f4(v3(gxs[m]) + gPk);
+                gv_out[pp] = f4(gPk * c.dt);
+            }
+            const float d2 = c.dt * c.dt;
+            gg[0] = gPk.x * d2;
+            gg[1] = gPk.y * d2;
+            gg[2] = gPk.z * d2;
+        }
+    }
+    if (FIRST) block_sum<3>(gg, part + (size_t)tile * kParts + 2);
+}
+
+// Per-scene sums of the per-tile partials (tiles of a scene are contiguous),
+// added to the running totals.  One wave per scene.
+__global__ __launch_bounds__(64) void k_param_reduce(const uint2* __restrict__ tile_range,
+                                                    const float* __restrict__ part, float* __restrict__ gparam) {
+    const uint32_t s = blockIdx.x;
+    const uint2 tr = tile_range[s];
+    const int lane = threadIdx.x;
+    for (int k = 0; k < 5; ++k) {
+        float a = 0.0f;
+        for (uint32_t t = tr.x + lane; t < tr.y; t += 64) a += part[(size_t)t * kParts + k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        if (lane == 0) gparam[s * 5 + k] += a;
+    }
+}
+
+// One recorded substep (device arrays).
+struct TapeSlot {
+    uint32_t* perm = nullptr;
+    uint32_t* cnt = nullptr;
+    uint4* quads = nullptr;
+    uint32_t* tcnt = nullptr;
+    uint4* tquads = nullptr;
+    float4* P = nullptr;     // (K+1) x n: iterates, w = lambda_k (k < K) / density (k = K)
+    float4* aux = nullptr;   // K x n: (sum of gradients, denominator) of computeScaling
+    float4* fut = nullptr;   // K x n: pre-wall position, w = planes that moved it
+    float4* vu = nullptr;    // velocity from updateVelocities
+    float4* om = nullptr;    // omega from applyViscosity
+};
+
+size_t quad_words(size_t n) { return ((n + 63) / 64) * 64 * (size_t)kQuads; }
+
+uint64_t slot_bytes(size_t n, size_t K) {
+    return 4 * n * 3 + 16 * quad_words(n) * 2 + 16 * n * ((K + 1) + 2 * K + 2) + 256 * 8;
+}
+
+}  // namespace
+}  // namespace pbf
+
+struct pbf_adjoint {
+    pbf_ctx* ctx = nullptr;
+    uint32_t T = 0, C = 0, segs = 0, K = 0;
+    std::vector<pbf::TapeSlot> slots;
+    std::vector<void*> allocs;
+    float4* ckpt = nullptr;            // segs x {x, v} x n
+    uint32_t *ids = nullptr, *ids_tmp = nullptr;
+    float4 *gx = nullptr, *gv = nullptr, *gx2 = nullptr, *gv2 = nullptr, *gom = nullptr, *gxs = nullptr;
+    float4 *z = nullptr, *z2 = nullptr;
+    float* zb = nullptr;
+    float* part = nullptr;
+    float* gparam = nullptr;
+    uint2* tile_range = nullptr;
+    unsigned long long* notes = nullptr;   // C + 1 snapshots of the dropped-friends counter
+    uint32_t* flags = nullptr;
+    bool have_forward = false;
+    int recorded = -1;                     // segment whose substeps the slots hold
+    double fwd_ms = 0.0, bwd_ms = 0.0, replay_ms = 0.0;
+};
+
+namespace pbf {
+namespace {
+
+template <class T>
+hipError_t aalloc(pbf_adjoint* a, T** p, size_t count) {
+    const hipError_t e = hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1));
+    if (e == hipSuccess) a->allocs.push_back((void*)*p);
+    return e;
+}
+
+void free_adjoint(pbf_adjoint* a) {
+    for (void* p : a->allocs) (void)hipFree(p);
+    a->allocs.clear();
+}
+
+dim3 grid_of(uint32_t n) { return dim3((n + 255) / 256); }
+
+// One substep with the forward kernels, recording into slot t.  The context's
+// buffers are pointed at the slot for the arrays the slot keeps and restored
+// afterwards; the rotation of x and v ends as after a plain step.
+int tape_step(pbf_adjoint* a, const TapeSlot& t, unsigned long long* note) {
+    pbf_ctx* ctx = a->ctx;
+    const size_t n = ctx->n;
+    const int K = (int)a->K;
+    uint32_t* const perm0 = ctx->perm;
+    uint32_t* const cnt0 = ctx->nb_cnt;
+    uint4* const quads0 = ctx->nb_quads;
+    float4* const xs0 = ctx->xs[0];
+    float4* const xs1 = ctx->xs[1];
+    float4* const om0 = ctx->omega;
+    float4* const v0 = ctx->v;
+    auto restore = [&]() {
+        ctx->perm = perm0;
+        ctx->nb_cnt = cnt0;
+        ctx->nb_quads = quads0;
+        ctx->xs[0] = xs0;
+        ctx->xs[1] = xs1;
+        ctx->omega = om0;
+    };
+    ctx->pause_now = 0;
+    hipError_t e = launch_clear(ctx);
+    if (e == hipSuccess) e = launch_predict_hash(ctx, ctx->cells_hold_keys);
+    ctx->perm = t.perm;
+    if (e == hipSuccess) e = launch_sort(ctx);
+    ctx->xs[0] = t.P;
+    if (e == hipSuccess) e = launch_gather(ctx);
+    if (e == hipSuccess) e = launch_cells(ctx);
+    ctx->cells_hold_keys = true;
+    ctx->order_sorted = true;
+    ctx->nb_cnt = t.cnt;
+    ctx->nb_quads = t.quads;
+    if (e == hipSuccess) e = launch_friends(ctx);
+    if (e == hipSuccess) {
+        const BlockTable& bt = ctx->bt[2];
+        hipLaunchKernelGGL(k_transpose, dim3(bt.count), dim3(kAdjThreads), 0, ctx->stream, ctx->sc_d, bt.d, t.P,
+                           t.cnt, ctx->nb_list, reinterpret_cast<const uint2*>(ctx->cells), t.tcnt, t.tquads,
+                           a->flags);
+        hipLaunchKernelGGL(k_note_drops, dim3(1), dim3(64), 0, ctx->stream, ctx->err, note);
+        e = hipGetLastError();
+    }
+    for (int k = 0; k < K && e == hipSuccess; ++k) {
+        ctx->xs[ctx->xs_cur] = t.P + (size_t)k * n;
+        ctx->xs[1 - ctx->xs_cur] = t.P + (size_t)(k + 1) * n;
+        e = launch_lambda(ctx, k, t.aux + (size_t)k * n);
+        if (k == K - 1) ctx->v = t.vu;   // updateVelocities writes Vu into the slot
+        if (e == hipSuccess) e = launch_delta(ctx, k, k == K - 1, t.fut + (size_t)k * n);
+    }
+    ctx->omega = t.om;
+    if (e == hipSuccess) e = launch_viscosity(ctx);   // v <- v_alt (owned), v_alt <- slot
+    if (e == hipSuccess) e = launch_vorticity(ctx);
+    ctx->v_alt = v0;
+    restore();
+    if (e != hipSuccess) return fail(ctx, PBF_E_HIP, "recording a substep: %s", hipGetErrorString(e));
+    return PBF_OK;
+}
+
+int ids_step(pbf_adjoint* a, const uint32_t* perm) {
+    pbf_ctx* ctx = a->ctx;
+    hipLaunchKernelGGL(k_ids_gather, grid_of(ctx->n), dim3(256), 0, ctx->stream, perm, a->ids, a->ids_tmp, ctx->n);
+    std::swap(a->ids, a->ids_tmp);
+    HIP_TRY(ctx, hipGetLastError());
+    return PBF_OK;
+}
+
+// Replays segment g from its checkpoint, recording every substep.
+int record_segment(pbf_adjoint* a, uint32_t g) {
+    pbf_ctx* ctx = a->ctx;
+    const size_t n = ctx->n;
+    const uint32_t s0 = g * a->C, s1 = std::min(a->T, s0 + a->C);
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->x, a->ckpt + (size_t)g * 2 * n, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->v, a->ckpt + ((size_t)g * 2 + 1) * n, 16 * n, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    hipLaunchKernelGGL(k_note_drops, dim3(1), dim3(64), 0, ctx->stream, ctx->err, a->notes);
+    for (uint32_t s = s0; s < s1; ++s) {
+        const int rc = tape_step(a, a->slots[s - s0], a->notes + (s - s0) + 1);
+        if (rc) return rc;
+    }
+    a->recorded = (int)g;
+    return PBF_OK;
+}
+
+// The recorded segment g is differentiable by the transposed lists: no
+// friend dropped and no bucket visited twice (checked on the host after the
+// device work has finished).
+int check_segment(pbf_adjoint* a, uint32_t g) {
+    pbf_ctx* ctx = a->ctx;
+    const uint32_t len = std::min(a->T, (g + 1) * a->C) - g * a->C;
+    std::vector<unsigned long long> notes(len + 1);
+    uint32_t flags = 0;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(notes.data(), a->notes, 8 * (len + 1), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(&flags, a->flags, 4, hipMemcpyDeviceToHost));
+    for (uint32_t t = 0; t < len; ++t)
+        if (notes[t + 1] != notes[t])
+            return fail(ctx, PBF_E_STATE,
+                        "substep %u dropped %llu friends (50 per circle): its friend lists are not symmetric, "
+                        "which the adjoint requires",
+                        g * a->C + t, notes[t + 1] - notes[t]);
+    if (flags & kDupFlag)
+        return fail(ctx, PBF_E_STATE,
+                    "a particle's 27 neighbour cells share a hash bucket (gridBufSize folds them); the adjoint "
+                    "requires each bucket visited once -- use a power-of-two gridBufSize >= 64");
+    return PBF_OK;
+}
+
+int backward_substep(pbf_adjoint* a, const TapeSlot& t) {
+    pbf_ctx* ctx = a->ctx;
+    const size_t n = ctx->n;
+    const int K = (int)a->K;
+    const BlockTable& bt = ctx->bt[2];
+    const dim3 grid(bt.count), block(kAdjThreads);
+    hipStream_t st = ctx->stream;
+    const float4* PK = t.P + (size_t)K * n;
+    hipLaunchKernelGGL(k_vort_b1, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.om, t.cnt, t.quads, a->gv, a->z,
+                       a->gom, a->part);
+    hipLaunchKernelGGL(k_vort_b2, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.om, t.cnt, t.quads, t.tcnt, t.tquads,
+                       a->z, a->gom, a->gx);
+    hipLaunchKernelGGL(k_xsph_b, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.vu, t.cnt, t.quads, t.tcnt, t.tquads,
+                       a->gv, a->gom, a->gx, a->gxs, a->z, t.fut + (size_t)(K - 1) * n, a->part);
+    for (int k = K - 1; k >= 0; --k) {
+        const float4* Pk = t.P + (size_t)k * n;
+        hipLaunchKernelGGL(k_delta_b, grid, block, 0, st, ctx->sc_d, bt.d, Pk, t.aux + (size_t)k * n, t.cnt,
+                           t.quads, t.tcnt, t.tquads, a->z, a->gx, a->z2, a->zb);
+        if (k > 0) {
+            hipLaunchKernelGGL(k_lambda_b<false>, grid, block, 0, st, ctx->sc_d, bt.d, Pk, t.cnt, t.quads, t.tcnt,
+                               t.tquads, a->z2, a->zb, a->gx, t.fut + (size_t)(k - 1) * n, a->z, t.perm, a->gxs,
+                               a->gx2, a->gv2, a->part, ctx->n);
+        } else {
+            hipLaunchKernelGGL(k_lambda_b<true>, grid, block, 0, st, ctx->sc_d, bt.d, Pk, t.cnt, t.quads, t.tcnt,
+                               t.tquads, a->z2, a->zb, a->gx, nullptr, a->z, t.perm, a->gxs, a->gx2, a->gv2,
+                               a->part, ctx->n);
+        }
+    }
+    hipLaunchKernelGGL(k_param_reduce, dim3(ctx->S), dim3(64), 0, st, a->tile_range, a->part, a->gparam);
+    HIP_TRY(ctx, hipGetLastError());
+    std::swap(a->gx, a->gx2);
+    std::swap(a->gv, a->gv2);
+    return PBF_OK;
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+}  // namespace pbf
+
+extern "C" {
+
+int pbf_set_scene_coefficients(pbf_ctx* ctx, uint32_t scene, float viscosity, float vorticity,
+                               const float gravity[3]) {
+    if (!ctx) return PBF_E_INVALID;
+    if (scene >= ctx->S) return pbf::fail(ctx, PBF_E_INVALID, "scene %u out of range (%u scenes)", scene, ctx->S);
+    if (!gravity) return pbf::fail(ctx, PBF_E_INVALID, "gravity is NULL");
+    pbf::SceneConsts& c = ctx->sc[scene];
+    c.viscosity = viscosity;
+    c.vorticity = vorticity;
+    c.g0 = gravity[0];
+    c.g1 = gravity[1];
+    c.g2 = gravity[2];
+    ctx->P[scene].viscosityFactor = viscosity;
+    ctx->P[scene].vorticityFactor = vorticity;
+    ctx->opt[scene].use_gravity_vector = 1;
+    for (int k = 0; k < 3; ++k) ctx->opt[scene].gravity[k] = gravity[k];
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(ctx->sc_d + scene, &c, sizeof c, hipMemcpyHostToDevice));
+    return PBF_OK;
+}
+
+int pbf_get_scene_coefficients(pbf_ctx* ctx, uint32_t scene, float* out5) {
+    if (!ctx || !out5) return PBF_E_INVALID;
+    if (scene >= ctx->S) return pbf::fail(ctx, PBF_E_INVALID, "scene %u out of range (%u scenes)", scene, ctx->S);
+    const pbf::SceneConsts& c = ctx->sc[scene];
+    out5[0] = c.viscosity;
+    out5[1] = c.vorticity;
+    out5[2] = c.g0;
+    out5[3] = c.g1;
+    out5[4] = c.g2;
+    return PBF_OK;
+}
+
+int pbf_adjoint_tape_bytes(pbf_ctx* ctx, uint64_t* per_substep, uint64_t* per_checkpoint) {
+    if (!ctx) return PBF_E_INVALID;
+    if (per_substep) *per_substep = pbf::slot_bytes(ctx->n, ctx->iters);
+    if (per_checkpoint) *per_checkpoint = 32ull * ctx->n;
+    return PBF_OK;
+}
+
+int pbf_adjoint_plan(pbf_ctx* ctx, uint32_t substeps, uint64_t budget_bytes, uint32_t* interval) {
+    if (!ctx || !interval || substeps == 0) return PBF_E_INVALID;
+    const uint64_t slot = pbf::slot_bytes(ctx->n, ctx->iters), ck = 32ull * ctx->n;
+    const uint64_t fixed = 4ull * 2 * ctx->n + 16ull * 8 * ctx->n + 4ull * ctx->n;   // ids, gradients, coefficients
+    for (uint32_t C = substeps; C >= 1; --C) {
+        const uint64_t segs = (substeps + C - 1) / C;
+        if (fixed + C * slot + segs * ck <= budget_bytes) {
+            *interval = C;
+            return PBF_OK;
+        }
+    }
+    return pbf::fail(ctx, PBF_E_NOMEM, "one recorded substep (%llu B) does not fit in %llu B",
+                     (unsigned long long)slot, (unsigned long long)budget_bytes);
+}
+
+int pbf_adjoint_create(pbf_ctx* ctx, uint32_t substeps, uint32_t interval, pbf_adjoint** out) {
+    if (!ctx || !out) return PBF_E_INVALID;
+    *out = nullptr;
+    if (substeps == 0 || interval == 0) return pbf::fail(ctx, PBF_E_INVALID, "substeps and interval must be >= 1");
+    pbf_adjoint* a = new (std::nothrow) pbf_adjoint();
+    if (!a) return pbf::fail(ctx, PBF_E_NOMEM, "host allocation failed");
+    a->ctx = ctx;
+    a->T = substeps;
+    a->C = std::min(interval, substeps);
+    a->segs = (a->T + a->C - 1) / a->C;
+    a->K = ctx->iters;
+    const size_t n = ctx->n, K = a->K, qw = pbf::quad_words(n);
+    auto bail = [&](hipError_t e) {
+        const int code = e == hipErrorOutOfMemory ? PBF_E_NOMEM : PBF_E_HIP;
+        pbf::fail(ctx, code, "adjoint allocation: %s", hipGetErrorString(e));
+        pbf::free_adjoint(a);
+        delete a;
+        return code;
+    };
+#define ADJ_TRY(call)                          \
+    do {                                       \
+        const hipError_t e_ = (call);          \
+        if (e_ != hipSuccess) return bail(e_); \
+    } while (0)
+    a->slots.resize(a->C);
+    for (pbf::TapeSlot& t : a->slots) {
+        ADJ_TRY(pbf::aalloc(a, &t.perm, n));
+        ADJ_TRY(pbf::aalloc(a, &t.cnt, n));
+        ADJ_TRY(pbf::aalloc(a, &t.tcnt, n));
+        ADJ_TRY(pbf::aalloc(a, &t.quads, qw));
+        ADJ_TRY(pbf::aalloc(a, &t.tquads, qw));
+        ADJ_TRY(pbf::aalloc(a, &t.P, (K + 1) * n));
+        ADJ_TRY(pbf::aalloc(a, &t.aux, K * n));
+        ADJ_TRY(pbf::aalloc(a, &t.fut, K * n));
+        ADJ_TRY(pbf::aalloc(a, &t.vu, n));
+        ADJ_TRY(pbf::aalloc(a, &t.om, n));
+        // updateVelocities writes only xyz of the recorded velocity: w stays 0
+        ADJ_TRY(hipMemset(t.vu, 0, 16 * std::max<size_t>(n, 1)));
+    }
+    ADJ_TRY(pbf::aalloc(a, &a->ckpt, (size_t)a->segs * 2 * n));
+    ADJ_TRY(pbf::aalloc(a, &a->ids, n));
+    ADJ_TRY(pbf::aalloc(a, &a->ids_tmp, n));
+    ADJ_TRY(pbf::aalloc(a, &a->gx, n));
+    ADJ_TRY(pbf::aalloc(a, &a->gv, n));
+    ADJ_TRY(pbf::aalloc(a, &a->gx2, n));
+    ADJ_TRY(pbf::aalloc(a, &a->gv2, n));
+    ADJ_TRY(pbf::aalloc(a, &a->gom, n));
+    ADJ_TRY(pbf::aalloc(a, &a->gxs, n));
+    ADJ_TRY(pbf::aalloc(a, &a->z, n));
+    ADJ_TRY(pbf::aalloc(a, &a->z2, n));
+    ADJ_TRY(pbf::aalloc(a, &a->zb, n));
+    const pbf::BlockTable& bt = ctx->bt[2];
+    ADJ_TRY(pbf::aalloc(a, &a->part, (size_t)bt.count * pbf::kParts));
+    ADJ_TRY(pbf::aalloc(a, &a->gparam, (size_t)ctx->S * 5));
+    ADJ_TRY(pbf::aalloc(a, &a->tile_range, ctx->S));
+    ADJ_TRY(pbf::aalloc(a, &a->notes, (size_t)a->C + 1));
+    ADJ_TRY(pbf::aalloc(a, &a->flags, 1));
+    // tiles of the 256-particle block table per scene
+    std::vector<uint2> tr(ctx->S);
+    uint32_t tile = 0;
+    for (uint32_t s = 0; s < ctx->S; ++s) {
+        const uint32_t tiles = (ctx->sc[s].count + 255) / 256;
+        tr[s] = make_uint2(tile, tile + tiles);
+        tile += tiles;
+    }
+    if (tile != bt.count) {
+        pbf::fail(ctx, PBF_E_INVALID, "internal: block table has %u tiles, expected %u", bt.count, tile);
+        pbf::free_adjoint(a);
+        delete a;
+        return PBF_E_INVALID;
+    }
+    ADJ_TRY(hipMemcpy(a->tile_range, tr.data(), sizeof(uint2) * ctx->S, hipMemcpyHostToDevice));
+    ADJ_TRY(hipMemset(a->flags, 0, 4));
+#undef ADJ_TRY
+    *out = a;
+    return PBF_OK;
+}
+
+int pbf_adjoint_forward(pbf_adjoint* a, const float* x0, const float* v0, float* xT, float* vT) {
+    if (!a) return PBF_E_INVALID;
+    pbf_ctx* ctx = a->ctx;
+    if (!x0 || !v0) return pbf::fail(ctx, PBF_E_INVALID, "x0 and v0 are required");
+    const double t0 = pbf::now_ms();
+    const size_t n = ctx->n;
+    a->have_forward = false;
+    if (n) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->x, x0, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->v, v0, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    ctx->have_state = true;
+    ctx->order_sorted = false;
+    if (ctx->x_base == nullptr) {
+        ctx->x_base = ctx->x;
+        ctx->v_base = ctx->v;
+    }
+    if (n == 0) return PBF_OK;
+    hipLaunchKernelGGL(pbf::k_iota, pbf::grid_of(ctx->n), dim3(256), 0, ctx->stream, a->ids, ctx->n);
+    HIP_TRY(ctx, hipMemsetAsync(a->flags, 0, 4, ctx->stream));
+    for (uint32_t g = 0; g < a->segs; ++g) {
+        HIP_TRY(ctx, hipMemcpyAsync(a->ckpt + (size_t)g * 2 * n, ctx->x, 16 * n, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(a->ckpt + ((size_t)g * 2 + 1) * n, ctx->v, 16 * n, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        const uint32_t s0 = g * a->C, s1 = std::min(a->T, s0 + a->C);
+        if (g + 1 == a->segs) {
+            hipLaunchKernelGGL(pbf::k_note_drops, dim3(1), dim3(64), 0, ctx->stream, ctx->err, a->notes);
+            for (uint32_t s = s0; s < s1; ++s) {
+                int rc = pbf::tape_step(a, a->slots[s - s0], a->notes + (s - s0) + 1);
+                if (!rc) rc = pbf::ids_step(a, a->slots[s - s0].perm);
+                if (rc) return rc;
+            }
+            a->recorded = (int)g;
+        } else {
+            for (uint32_t s = s0; s < s1; ++s) {
+                int rc = pbf_step(ctx, 0);
+                if (!rc) rc = pbf::ids_step(a, ctx->perm);
+                if (rc) return rc;
+            }
+        }
+    }
+    if (xT || vT)
+        hipLaunchKernelGGL(pbf::k_scatter_state, pbf::grid_of(ctx->n), dim3(256), 0, ctx->stream, a->ids, ctx->x,
+                           ctx->v, reinterpret_cast<float4*>(xT), reinterpret_cast<float4*>(vT), ctx->n);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    a->have_forward = true;
+    a->fwd_ms = pbf::now_ms() - t0;
+    return PBF_OK;
+}
+
+int pbf_adjoint_backward(pbf_adjoint* a, const float* gxT, const float* gvT, float* gx0, float* gv0,
+                         float* gcoef) {
+    if (!a) return PBF_E_INVALID;
+    pbf_ctx* ctx = a->ctx;
+    if (!a->have_forward) return pbf::fail(ctx, PBF_E_STATE, "pbf_adjoint_backward before pbf_adjoint_forward");
+    const double t0 = pbf::now_ms();
+    double replay = 0.0;
+    const size_t n = ctx->n;
+    if (n == 0) return PBF_OK;
+    hipLaunchKernelGGL(pbf::k_gather_grads, pbf::grid_of(ctx->n), dim3(256), 0, ctx->stream, a->ids,
+                       reinterpret_cast<const float4*>(gxT), reinterpret_cast<const float4*>(gvT), a->gx, a->gv,
+                       ctx->n);
+    hipLaunchKernelGGL(pbf::k_fill, dim3((ctx->S * 5 + 255) / 256), dim3(256), 0, ctx->stream, a->gparam,
+                       (size_t)ctx->S * 5);
+    HIP_TRY(ctx, hipGetLastError());
+    for (int g = (int)a->segs - 1; g >= 0; --g) {
+        if (a->recorded != g) {
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            const double r0 = pbf::now_ms();
+            const int rc = pbf::record_segment(a, (uint32_t)g);
+            if (rc) return rc;
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            replay += pbf::now_ms() - r0;
+        }
+        int rc = pbf::check_segment(a, (uint32_t)g);
+        if (rc) return rc;
+        const uint32_t s0 = (uint32_t)g * a->C, s1 = std::min(a->T, s0 + a->C);
+        for (uint32_t s = s1; s-- > s0;) {
+            rc = pbf::backward_substep(a, a->slots[s - s0]);
+            if (rc) return rc;
+        }
+    }
+    if (gx0) HIP_TRY(ctx, hipMemcpyAsync(gx0, a->gx, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    if (gv0) HIP_TRY(ctx, hipMemcpyAsync(gv0, a->gv, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    if (gcoef)
+        HIP_TRY(ctx, hipMemcpyAsync(gcoef, a->gparam, 4 * 5 * (size_t)ctx->S, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    a->bwd_ms = pbf::now_ms() - t0;
+    a->replay_ms = replay;
+    return PBF_OK;
+}
+
+int pbf_adjoint_times(pbf_adjoint* a, double* forward_ms, double* backward_ms, double* replay_ms) {
+    if (!a) return PBF_E_INVALID;
+    if (forward_ms) *forward_ms = a->fwd_ms;
+    if (backward_ms) *backward_ms = a->bwd_ms;
+    if (replay_ms) *replay_ms = a->replay_ms;
+    return PBF_OK;
+}
+
+void pbf_adjoint_destroy(pbf_adjoint* a) {
+    if (!a) return;
+    if (a->ctx && a->ctx->stream) (void)hipStreamSynchronize(a->ctx->stream);
+    pbf::free_adjoint(a);
+    delete a;
+}
+
+}  // extern "C"

@@ -17,9 +17,10 @@
 #include "pbf_internal.h"
 
 namespace {
-
 std::string g_create_error;
+}  // namespace
 
+namespace pbf {
 int fail(pbf_ctx* ctx, int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -32,12 +33,11 @@ int fail(pbf_ctx* ctx, int code, const char* fmt, ...) {
         g_create_error = buf;
     return code;
 }
+}  // namespace pbf
 
-#define HIP_TRY(ctx, call)                                                                              \
-    do {                                                                                                \
-        hipError_t e_ = (call);                                                                         \
-        if (e_ != hipSuccess) return fail((ctx), PBF_E_HIP, "%s: %s", #call, hipGetErrorString(e_));    \
-    } while (0)
+namespace {
+
+using pbf::fail;
 
 // Simulation.cpp:167,179-180: the reference streams the double expressions
 // with std::showpoint at the default precision (6 significant digits) and a

@@ -121,6 +121,30 @@ __device__ __forceinline__ size_t sweep_quad(uint32_t i, int q) {
     return ((size_t)(i >> 6) * kQuads + (size_t)q) * 64 + (i & 63u);
 }
 
+// Walks the first m entries of particle i's list in quad layout, four per
+// 16-byte load (fn4 on a full quad, fn1 on each entry of the partial last one).
+template <class Fn4, class Fn1>
+__device__ __forceinline__ void for_quads(uint32_t i, int m, const uint4* __restrict__ quads, Fn4&& fn4, Fn1&& fn1) {
+    int q = 0;
+    for (; 4 * q + 4 <= m; ++q) fn4(quads[sweep_quad(i, q)]);
+    const int rest = m - 4 * q;
+    if (rest > 0) {
+        const uint4 jj = quads[sweep_quad(i, q)];
+        fn1(jj.x);
+        if (rest > 1) fn1(jj.y);
+        if (rest > 2) fn1(jj.z);
+    }
+}
+
+// The friends a sweep with threshold L_NUM/L_DEN processes.
+template <int L_NUM, int L_DEN, class Fn4, class Fn1>
+__device__ __forceinline__ void for_friends(uint32_t i, uint32_t packed, const uint4* __restrict__ quads, Fn4&& fn4,
+                                            Fn1&& fn1) {
+    int m;
+    (void)circles_processed<L_NUM, L_DEN>(packed, &m);
+    for_quads(i, m, quads, fn4, fn1);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs of the chip; this
 // maps dispatch index b of nb workgroups to a tile so that each XCD takes one
 // contiguous run of tiles.  Neighbouring particles of a tile then sit in the
@@ -132,36 +156,51 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
 }
 
 // Box wall planes: BouncePointQuad (compute_delta.cl:1-42) for an infinite
-// axis-aligned plane with mask 1.
-__device__ __forceinline__ float wall_lo(float f, float prev, float lo, float edge) {
+// axis-aligned plane with mask 1.  `hit` records whether the plane moved the
+// point (the adjoint replays the same branch).
+__device__ __forceinline__ float wall_lo(float f, float prev, float lo, float edge, bool& hit) {
     const float pa = lo + edge;
     const float dp = f - pa;
     if (dp > 0.0f) return f;
     const float nv = prev - f;
     if (fabsf(dp) > edge + nv) return f;
+    hit = true;
     return pa;
 }
 
-__device__ __forceinline__ float wall_hi(float f, float prev, float hi, float edge) {
+__device__ __forceinline__ float wall_hi(float f, float prev, float hi, float edge, bool& hit) {
     const float pa = hi - edge;
     const float dp = f - pa;
     if (dp < 0.0f) return f;
     const float nv = f - prev;
     if (fabsf(dp) > edge + nv) return f;
+    hit = true;
     return pa;
 }
 
-__device__ __forceinline__ void apply_walls(float& fx, float& fy, float& fz, const float4& prev, const SceneConsts& c) {
+// Edge offset of the force zone (compute_delta.cl:165-166 without its
+// scene-scale "1.0 +" term).
+__device__ __forceinline__ float wall_edge(float fx, float fy, float fz, const SceneConsts& c) {
     const float nx = noise_sin(fx * 5.0f / c.h);
     const float ny = noise_sin(fy * 5.0f / c.h);
     const float nz = noise_sin(fz * 5.0f / c.h);
-    const float edge = (3.0f + nx + ny + nz) * c.h * 0.03f;
-    fx = wall_lo(fx, prev.x, c.lo0, edge);
-    fx = wall_hi(fx, prev.x, c.hi0, edge);
-    fy = wall_lo(fy, prev.y, c.lo1, edge);
-    fy = wall_hi(fy, prev.y, c.hi1, edge);
-    fz = wall_lo(fz, prev.z, c.lo2, edge);
-    fz = wall_hi(fz, prev.z, c.hi2, edge);
+    return (3.0f + nx + ny + nz) * c.h * 0.03f;
+}
+
+// Planes in the order x-lo, x-hi, y-lo, y-hi, z-lo, z-hi.  Returns which
+// moved the point: bit 2a (lo) and 2a+1 (hi) of axis a.
+__device__ __forceinline__ uint32_t apply_walls(float& fx, float& fy, float& fz, const float4& prev,
+                                                const SceneConsts& c) {
+    const float edge = wall_edge(fx, fy, fz, c);
+    bool h0 = false, h1 = false, h2 = false, h3 = false, h4 = false, h5 = false;
+    fx = wall_lo(fx, prev.x, c.lo0, edge, h0);
+    fx = wall_hi(fx, prev.x, c.hi0, edge, h1);
+    fy = wall_lo(fy, prev.y, c.lo1, edge, h2);
+    fy = wall_hi(fy, prev.y, c.hi1, edge, h3);
+    fz = wall_lo(fz, prev.z, c.lo2, edge, h4);
+    fz = wall_hi(fz, prev.z, c.hi2, edge, h5);
+    return (uint32_t)h0 | ((uint32_t)h1 << 1) | ((uint32_t)h2 << 2) | ((uint32_t)h3 << 3) | ((uint32_t)h4 << 4) |
+           ((uint32_t)h5 << 5);
 }
 
 }  // namespace pbf

@@ -409,25 +409,10 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // Friend sweeps.  Each walks the friends of the circles it processes in the
-// reference's order (circle-major, then scan order), four per 16-byte load:
-// the four gathers are issued together, then the updates run in order.
+// reference's order (circle-major, then scan order), four per 16-byte load
+// (for_friends, pbf_device.h): the four gathers are issued together, then the
+// updates run in order.
 // ---------------------------------------------------------------------------
-
-template <int L_NUM, int L_DEN, class Fn4, class Fn1>
-__device__ __forceinline__ void for_friends(uint32_t i, uint32_t packed, const uint4* __restrict__ quads, Fn4&& fn4,
-                                            Fn1&& fn1) {
-    int m;
-    (void)circles_processed<L_NUM, L_DEN>(packed, &m);
-    int q = 0;
-    for (; 4 * q + 4 <= m; ++q) fn4(quads[sweep_quad(i, q)]);
-    const int rest = m - 4 * q;
-    if (rest > 0) {
-        const uint4 jj = quads[sweep_quad(i, q)];
-        fn1(jj.x);
-        if (rest > 1) fn1(jj.y);
-        if (rest > 2) fn1(jj.z);
-    }
-}
 
 // compute_scaling.cl:1-113 + pack_data.cl (lambda into x*.w, in place: no
 // thread of this kernel reads another particle's w).
@@ -449,11 +434,13 @@ struct LambdaAcc {
     }
 };
 
-template <int BS>
+// TAPE also stores (sum of gradients, denominator) of every particle for the
+// adjoint (aux).
+template <int BS, bool TAPE>
 __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
                                               float4* __restrict__ xs, const uint32_t* __restrict__ nb_cnt,
                                               const uint4* __restrict__ nb_list, float* __restrict__ rho,
-                                              float* __restrict__ lambda, uint32_t n) {
+                                              float* __restrict__ lambda, float4* __restrict__ aux, uint32_t n) {
     const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
     const SceneConsts& c = scenes[be.x];
     const uint32_t i = be.y + threadIdx.x;
@@ -471,9 +458,11 @@ __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ s
     rho[i] = dens;
     const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
     const float C = (dens / c.rest_density) - 1.0f;
-    const float lam = -1.0f * C / (gsum / (c.rest_density * c.rest_density) + c.epsilon);
+    const float den = gsum / (c.rest_density * c.rest_density) + c.epsilon;
+    const float lam = -1.0f * C / den;
     lambda[i] = lam;
     reinterpret_cast<float*>(xs + i)[3] = lam;
+    if (TAPE) aux[i] = make_float4(a.gx, a.gy, a.gz, den);
 }
 
 // compute_delta.cl:44-186 + update_predicted.cl; on the last iteration also
@@ -497,12 +486,14 @@ struct DeltaAcc {
     }
 };
 
-template <int BS, bool LAST>
+// TAPE also stores the position before the walls and which planes moved it
+// (bits in w) for the adjoint (fut).
+template <int BS, bool LAST, bool TAPE>
 __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
                                              const float4* __restrict__ xs, float4* __restrict__ xs_next,
                                              float4* __restrict__ x, float4* __restrict__ v,
                                              const float* __restrict__ rho, const uint32_t* __restrict__ nb_cnt,
-                                             const uint4* __restrict__ nb_list, uint32_t n) {
+                                             const uint4* __restrict__ nb_list, float4* __restrict__ fut, uint32_t n) {
     const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
     const SceneConsts& c = scenes[be.x];
     const uint32_t i = be.y + threadIdx.x;
@@ -520,7 +511,10 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
     float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
     float fz = p.z + (-c.spiky * a.sz) / c.rest_density;
     const float4 prev = x[i];
-    if (c.walls) apply_walls(fx, fy, fz, prev, c);
+    const float ux = fx, uy = fy, uz = fz;
+    uint32_t hits = 0;
+    if (c.walls) hits = apply_walls(fx, fy, fz, prev, c);
+    if (TAPE) fut[i] = make_float4(ux, uy, uz, __uint_as_float(hits));
     const float dx = fx - p.x, dy = fy - p.y, dz = fz - p.z;
     float4 o;
     o.x = p.x + dx;
@@ -713,32 +707,43 @@ hipError_t launch_friends(pbf_ctx* ctx) {
     return hipGetLastError();
 }
 
-hipError_t launch_lambda(pbf_ctx* ctx, int iter) {
+hipError_t launch_lambda(pbf_ctx* ctx, int iter, float4* aux) {
     track_begin(ctx, "computeScaling", iter);
     const BlockTable& t = table_for(ctx, F_LAMBDA);
-    PBF_DISPATCH_BS(ctx->bs_log2[F_LAMBDA],
-                    hipLaunchKernelGGL(k_lambda<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d,
-                                       ctx->xs[ctx->xs_cur], ctx->nb_cnt, ctx->nb_quads, ctx->rho, ctx->lambda,
-                                       ctx->n));
+    if (aux) {
+        PBF_DISPATCH_BS(ctx->bs_log2[F_LAMBDA],
+                        hipLaunchKernelGGL((k_lambda<BS, true>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
+                                           t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt, ctx->nb_quads, ctx->rho,
+                                           ctx->lambda, aux, ctx->n));
+    } else {
+        PBF_DISPATCH_BS(ctx->bs_log2[F_LAMBDA],
+                        hipLaunchKernelGGL((k_lambda<BS, false>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
+                                           t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt, ctx->nb_quads, ctx->rho,
+                                           ctx->lambda, aux, ctx->n));
+    }
     track_end(ctx);
     return hipGetLastError();
 }
 
-hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last) {
+hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last, float4* fut) {
     track_begin(ctx, "computeDelta", iter);
     const int cur = ctx->xs_cur;
     const BlockTable& t = table_for(ctx, F_DELTA);
-    if (last) {
-        PBF_DISPATCH_BS(ctx->bs_log2[F_DELTA],
-                        hipLaunchKernelGGL((k_delta<BS, true>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
-                                           t.d, ctx->xs[cur], ctx->xs[1 - cur], ctx->x, ctx->v, ctx->rho, ctx->nb_cnt,
-                                           ctx->nb_quads, ctx->n));
+#define PBF_LAUNCH_DELTA(LASTV, TAPEV)                                                                            \
+    PBF_DISPATCH_BS(ctx->bs_log2[F_DELTA],                                                                       \
+                    hipLaunchKernelGGL((k_delta<BS, LASTV, TAPEV>), dim3(t.count), dim3(BS), 0, ctx->stream,      \
+                                       ctx->sc_d, t.d, ctx->xs[cur], ctx->xs[1 - cur], ctx->x, ctx->v, ctx->rho,  \
+                                       ctx->nb_cnt, ctx->nb_quads, fut, ctx->n))
+    if (last && fut) {
+        PBF_LAUNCH_DELTA(true, true);
+    } else if (last) {
+        PBF_LAUNCH_DELTA(true, false);
+    } else if (fut) {
+        PBF_LAUNCH_DELTA(false, true);
     } else {
-        PBF_DISPATCH_BS(ctx->bs_log2[F_DELTA],
-                        hipLaunchKernelGGL((k_delta<BS, false>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
-                                           t.d, ctx->xs[cur], ctx->xs[1 - cur], ctx->x, ctx->v, ctx->rho, ctx->nb_cnt,
-                                           ctx->nb_quads, ctx->n));
+        PBF_LAUNCH_DELTA(false, false);
     }
+#undef PBF_LAUNCH_DELTA
     track_end(ctx);
     ctx->xs_cur = 1 - cur;
     return hipGetLastError();

@@ -71,6 +71,12 @@ struct GraphEntry {
 
 }  // namespace pbf
 
+#define HIP_TRY(ctx, call)                                                                                   \
+    do {                                                                                                     \
+        hipError_t e_ = (call);                                                                              \
+        if (e_ != hipSuccess) return pbf::fail((ctx), PBF_E_HIP, "%s: %s", #call, hipGetErrorString(e_));    \
+    } while (0)
+
 struct pbf_ctx {
     std::vector<pbf_params> P;        // per scene
     std::vector<pbf_options> opt;     // per scene
@@ -140,10 +146,15 @@ hipError_t launch_sort(pbf_ctx* ctx);
 hipError_t launch_gather(pbf_ctx* ctx);
 hipError_t launch_cells(pbf_ctx* ctx);
 hipError_t launch_friends(pbf_ctx* ctx);
-hipError_t launch_lambda(pbf_ctx* ctx, int iter);
-hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last);
+// aux / fut: per-particle records for the adjoint (nullptr: plain step)
+hipError_t launch_lambda(pbf_ctx* ctx, int iter, float4* aux = nullptr);
+hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last, float4* fut = nullptr);
 hipError_t launch_viscosity(pbf_ctx* ctx);
 hipError_t launch_vorticity(pbf_ctx* ctx);
+
+// Records a formatted error on ctx (or for the next pbf_create when ctx is
+// NULL) and returns code.
+int fail(pbf_ctx* ctx, int code, const char* fmt, ...);
 
 // Profiling brackets (no-ops unless ctx->profiling).
 void track_begin(pbf_ctx* ctx, const char* name, int index = -1);
@@ -152,3 +163,9 @@ void collect_timings(pbf_ctx* ctx);
 
 inline const BlockTable& table_for(const pbf_ctx* ctx, Family f) { return ctx->bt[ctx->bs_log2[f] - 6]; }
 }  // namespace pbf
+
+#define HIP_TRY(ctx, call)                                                                                   \
+    do {                                                                                                     \
+        hipError_t e_ = (call);                                                                              \
+        if (e_ != hipSuccess) return pbf::fail((ctx), PBF_E_HIP, "%s: %s", #call, hipGetErrorString(e_));    \
+    } while (0)

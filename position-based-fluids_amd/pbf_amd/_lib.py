@@ -166,6 +166,17 @@ _SIGNATURES = {
     "pbf_error_flags": (ctypes.c_int, [_VP, _UP, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),
     "pbf_destroy": (None, [_VP]),
     "pbf_last_error": (ctypes.c_char_p, [_VP]),
+    # include/pbf/pbf_adjoint.h
+    "pbf_set_scene_coefficients": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, _FP]),
+    "pbf_get_scene_coefficients": (ctypes.c_int, [_VP, ctypes.c_uint32, _FP]),
+    "pbf_adjoint_tape_bytes": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "pbf_adjoint_plan": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint64, _UP]),
+    "pbf_adjoint_create": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_VP)]),
+    "pbf_adjoint_forward": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "pbf_adjoint_backward": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "pbf_adjoint_times": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]),
+    "pbf_adjoint_destroy": (None, [_VP]),
 }
 
 
@@ -177,6 +188,14 @@ def lib() -> ctypes.CDLL:
             raise LibraryMissing(
                 f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')"
             )
+        # One HIP runtime per process: PyTorch-ROCm bundles its own
+        # libamdhip64.so.7.  Loaded first, it is the one this library binds
+        # to (same soname); loaded after this library it would be a second
+        # runtime, so import it first whenever it is installed.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(L, name)

@@ -22,6 +22,7 @@ HOST = PKG_ROOT / "host"
 SIM_LIB_PATH = LIB_DIR / "libpbf_sim.so"
 BIN_DIR = PKG_ROOT / "bin"
 RUN_PATH = BIN_DIR / "pbf_run"
+TORCH_EXT_PATH = PKG_ROOT / "pbf_amd" / "pbf_torch.so"
 
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -39,7 +40,7 @@ HIP_FLAGS = [
     f"-I{REPO_ROOT / 'include'}",
 ]
 
-SOURCES = ["pbf_sort.hip", "pbf_forward.hip", "pbf_capi.hip", "pbf_probe.hip", "pbf_params.cpp"]
+SOURCES = ["pbf_sort.hip", "pbf_forward.hip", "pbf_capi.hip", "pbf_adjoint.hip", "pbf_probe.hip", "pbf_params.cpp"]
 
 
 def hipcc() -> str:
@@ -70,14 +71,16 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
         for f in futs:
             f.result()
     tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,libpbf_amd.so", *map(str, objs),
+           "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB_PATH)
     build_host()
+    build_torch_ext()
     if verbose:
-        print(f"built {LIB_PATH}, {SIM_LIB_PATH}, {RUN_PATH}")
+        print(f"built {LIB_PATH}, {SIM_LIB_PATH}, {RUN_PATH}, {TORCH_EXT_PATH}")
     return LIB_PATH
 
 
@@ -100,6 +103,33 @@ def build_host() -> None:
     if r.returncode != 0:
         raise RuntimeError(f"pbf_run build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, RUN_PATH)
+
+
+def build_torch_ext() -> None:
+    """The libtorch autograd Function (csrc/pbf_torch.cpp) as the Python
+    module pbf_amd.pbf_torch, linked against libpbf_amd.so (host code only:
+    the GPU work is in the library)."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+
+    cxx = shutil.which("g++") or "g++"
+    tlib = ce.library_paths()[0]
+    inc = [f"-I{p}" for p in ce.include_paths()] + [f"-I{sysconfig.get_paths()['include']}",
+                                                    f"-I{Path(hipcc()).resolve().parent.parent / 'include'}",
+                                                    f"-I{REPO_ROOT / 'include'}"]
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    tmp = TORCH_EXT_PATH.with_suffix(".so.tmp")
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-w", *inc, "-DTORCH_EXTENSION_NAME=pbf_torch",
+           "-DTORCH_API_INCLUDE_EXTENSION_H", f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-D__HIP_PLATFORM_AMD__=1",
+           "-DUSE_ROCM=1", str(CSRC / "pbf_torch.cpp"), "-o", str(tmp), f"-L{LIB_DIR}", "-lpbf_amd", f"-L{tlib}",
+           "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_python", f"-Wl,-rpath,$ORIGIN/../lib",
+           f"-Wl,-rpath,{tlib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"torch extension build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, TORCH_EXT_PATH)
 
 
 if __name__ == "__main__":

@@ -139,6 +139,21 @@ def dam_break(nx: int, ny: int, nz: int, h: float, box, *, seed: int = 0, **kw) 
     return Scene(p, default_options(), x, v, name=f"dam{nx}x{ny}x{nz}_h{h}")
 
 
+def inverse_problem(nx: int = 64, ny: int = 32, nz: int = 32, h: float = 0.1, sigma: float = 0.5, seed: int = 0,
+                    dt: float = 1.0 / 120.0, iters: int = 4) -> Scene:
+    """Block of nx*ny*nz particles at 0.5h in the corner of a box twice its
+    extent, with initial velocities N(0, sigma^2) per component (numpy PCG64
+    seeded with `seed`): the state whose rollout gives the target positions
+    of the inverse problem (default: 65,536 particles, h = 0.1)."""
+    x = dam_break_block(nx, ny, nz, h, seed=seed)
+    box = (nx * h, ny * h, nz * h)   # block spacing 0.5h: twice its extent
+    p = physical_params(x.shape[0], h, (0.0, 0.0, 0.0), box, dt=dt, iters=iters,
+                        grid_buf_size=morton_grid_size(box, h))
+    v = np.zeros_like(x)
+    v[:, :3] = np.random.default_rng(seed).normal(0.0, sigma, (x.shape[0], 3)).astype(np.float32)
+    return Scene(p, default_options(), x, v, name=f"inverse{nx}x{ny}x{nz}")
+
+
 def config(name: str, seed: int = 0) -> Scene:
     """Named dam-break workloads (dt = 1/120 s, 4 solver iterations).
 

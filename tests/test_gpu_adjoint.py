@@ -1,0 +1,219 @@
+"""GPU adjoint (include/pbf/pbf_adjoint.h) against the frozen-structure
+checker (oracle/pbf_adjoint_ref.cpp).
+
+The loss is L = mean_i |xT_i - target_i|^2 + 0.1 mean_i |vT_i|^2.  Its
+gradient from the GPU backward is compared, along random directions and per
+coefficient, with
+  * the directional derivative of the same binary32 rollout (dual numbers
+    with binary32 values and FP64 tangents), and
+  * FP64 central differences of every substep chained along that rollout,
+both to 1e-3 relative (the target; they agree far closer), and over a short
+horizon with FP64 central differences of the whole FP64 rollout.  Over long
+horizons the FP64 and binary32 rollouts of this scene separate (their
+difference grows about 5x per substep), so only the chained differences
+judge the binary32 gradient there.
+"""
+import numpy as np
+import pytest
+
+import pbf_amd
+from pbf_amd.adjoint import Rollout
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-3
+
+
+def _scene(nx=16, ny=16, nz=16, seed=1):
+    return pbf_amd.scenes.inverse_problem(nx, ny, nz, seed=seed)
+
+
+def _dev(a):
+    t = torch.zeros((a.shape[0], 4), dtype=torch.float32)
+    t[:, :3] = torch.from_numpy(np.ascontiguousarray(a[:, :3], np.float32))
+    return t.cuda().contiguous()
+
+
+class Problem:
+    """A rollout on the GPU, its loss gradient, and the frozen checker."""
+
+    def __init__(self, oracle, scene, steps, interval, seed=0):
+        self.scene = scene
+        self.steps = steps
+        self.sim = pbf_amd.Simulation(scene.params, scene.options)
+        self.roll = Rollout(self.sim, steps, interval)
+        cfg = oracle.cfg_from_params(scene.params, scene.options)
+        self.fr = oracle.FrozenRollout(cfg, scene.x, scene.v, steps)
+        n = scene.n
+        xT, vT = self.roll.forward_raw(_dev(scene.x), _dev(scene.v))
+        self.xT = xT.cpu().numpy()[:, :3]
+        self.vT = vT.cpu().numpy()[:, :3]
+        rng = np.random.default_rng(seed)
+        self.tgt = self.fr.xT32 + rng.normal(0, 0.01, (n, 3)).astype(np.float32)
+        self.gxT = (2.0 * (self.xT - self.tgt) / n).astype(np.float32)
+        self.gvT = (0.2 * self.vT / n).astype(np.float32)
+        gx0, gv0, gc = self.roll.backward_raw(_dev(self.gxT), _dev(self.gvT))
+        self.gx0 = gx0.cpu().numpy().astype(np.float64)
+        self.gv0 = gv0.cpu().numpy().astype(np.float64)
+        self.gc = gc.cpu().numpy().astype(np.float64)
+
+    def dL(self, dxT, dvT):
+        return float(np.sum(self.gxT * dxT) + np.sum(self.gvT * dvT))
+
+    def close(self):
+        self.roll.close()
+        self.sim.close()
+
+
+@pytest.fixture(scope="module")
+def problem(oracle):
+    p = Problem(oracle, _scene(), steps=24, interval=5)
+    yield p
+    p.close()
+
+
+def test_rollout_is_the_binary32_rollout(problem):
+    # the forward of the adjoint is the plain substep: bit for bit the oracle's
+    assert np.array_equal(problem.xT, problem.fr.xT32)
+    assert np.array_equal(problem.vT, problem.fr.vT32)
+    assert np.all(problem.gx0[:, 3] == 0.0) and np.all(problem.gv0[:, 3] == 0.0)
+
+
+def test_velocity_and_position_gradients_match_jvp_and_chained_fd(problem):
+    rng = np.random.default_rng(11)
+    n = problem.scene.n
+    dv = rng.normal(0, 1, (n, 3))
+    dx = rng.normal(0, 1e-2, (n, 3))
+    adj = float(np.sum(problem.gv0[:, :3] * dv) + np.sum(problem.gx0[:, :3] * dx))
+    _, _, dxT, dvT = problem.fr.jvp32(dx0=dx, dv0=dv)
+    jvp = problem.dL(dxT, dvT)
+    assert abs(adj - jvp) <= RTOL * abs(jvp), (adj, jvp)
+    fx, fv = problem.fr.tangent_fd(dx0=dx, dv0=dv)
+    fd = problem.dL(fx, fv)
+    assert abs(adj - fd) <= RTOL * abs(fd), (adj, fd)
+
+
+@pytest.mark.parametrize("k,name", [(0, "viscosity"), (1, "vorticity"), (2, "g0"), (3, "g1"), (4, "g2")])
+def test_coefficient_gradients_match_jvp_and_chained_fd(problem, k, name):
+    dc = np.zeros(5)
+    dc[k] = 1.0
+    _, _, dxT, dvT = problem.fr.jvp32(dcoef=dc)
+    jvp = problem.dL(dxT, dvT)
+    adj = problem.gc[0, k]
+    assert abs(adj - jvp) <= RTOL * abs(jvp), (name, adj, jvp)
+    fx, fv = problem.fr.tangent_fd(dcoef=dc)
+    fd = problem.dL(fx, fv)
+    assert abs(adj - fd) <= RTOL * abs(fd), (name, adj, fd)
+
+
+def test_short_horizon_matches_fp64_central_differences(oracle):
+    # over 4 substeps the FP64 rollout stays close to the binary32 one and
+    # the plain FP64 central difference of the whole rollout judges the
+    # gradient directly
+    p = Problem(oracle, _scene(seed=2), steps=4, interval=4, seed=1)
+    try:
+        rng = np.random.default_rng(5)
+        n = p.scene.n
+        dv = rng.normal(0, 1, (n, 3))
+        dcg = np.zeros(5)
+        dcg[2:] = rng.normal(0, 1, 3)
+        x0 = p.fr.x0.astype(np.float64)
+        v0 = p.fr.v0.astype(np.float64)
+        c0 = p.fr.coef0.astype(np.float64)
+        tgt = p.tgt.astype(np.float64)
+
+        def loss(xT, vT):
+            return np.mean(np.sum((xT - tgt) ** 2, 1)) + 0.1 * np.mean(np.sum(vT ** 2, 1))
+
+        for ddv, ddc, adj in ((dv, None, float(np.sum(p.gv0[:, :3] * dv))),
+                              (None, dcg, float(np.sum(p.gc[0] * dcg)))):
+            e = 1e-5
+            xp, vp = p.fr.rollout64(v0=v0 + (0 if ddv is None else e * ddv), coef=c0 + (0 if ddc is None else e * ddc))
+            xm, vm = p.fr.rollout64(v0=v0 - (0 if ddv is None else e * ddv), coef=c0 - (0 if ddc is None else e * ddc))
+            fd = (loss(xp, vp) - loss(xm, vm)) / (2 * e)
+            assert abs(adj - fd) <= RTOL * abs(fd), (adj, fd)
+    finally:
+        p.close()
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(oracle):
+    scene = _scene(8, 8, 8, seed=4)
+    out = []
+    for interval in (1, 3, 10):
+        sim = pbf_amd.Simulation(scene.params, scene.options)
+        roll = Rollout(sim, 10, interval)
+        xT, vT = roll.forward_raw(_dev(scene.x), _dev(scene.v))
+        g = roll.backward_raw(xT.clone(), vT.clone())
+        out.append([t.cpu().numpy() for t in g])
+        roll.close()
+        sim.close()
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert np.array_equal(a, b)
+
+
+def test_backward_can_run_twice(problem):
+    gx0, gv0, gc = problem.roll.backward_raw(_dev(problem.gxT), _dev(problem.gvT))
+    assert np.array_equal(gv0.cpu().numpy(), problem.gv0.astype(np.float32))
+    assert np.array_equal(gc.cpu().numpy(), problem.gc.astype(np.float32))
+
+
+def test_batched_adjoint_equals_per_scene_adjoints():
+    scenes = [_scene(8, 8, 8, seed=s) for s in (5, 6, 7)]
+    scenes[1] = pbf_amd.scenes.inverse_problem(12, 6, 6, seed=6)
+    steps = 6
+    batch = pbf_amd.BatchSimulation([s.params for s in scenes], [s.options for s in scenes])
+    roll = Rollout(batch, steps, 4)
+    x0 = np.concatenate([s.x for s in scenes])
+    v0 = np.concatenate([s.v for s in scenes])
+    xT, vT = roll.forward_raw(_dev(x0), _dev(v0))
+    gx, gv, gc = [t.cpu().numpy() for t in roll.backward_raw(xT.clone(), vT.clone())]
+    roll.close()
+    batch.close()
+    off = 0
+    for k, s in enumerate(scenes):
+        sim = pbf_amd.Simulation(s.params, s.options)
+        r = Rollout(sim, steps, 4)
+        a, b = r.forward_raw(_dev(s.x), _dev(s.v))
+        assert np.array_equal(a.cpu().numpy(), xT.cpu().numpy()[off:off + s.n])
+        hx, hv, hc = [t.cpu().numpy() for t in r.backward_raw(a.clone(), b.clone())]
+        assert np.array_equal(hx, gx[off:off + s.n])
+        assert np.array_equal(hv, gv[off:off + s.n])
+        assert np.array_equal(hc[0], gc[k])
+        r.close()
+        sim.close()
+        off += s.n
+
+
+def test_torch_autograd_function(problem):
+    from pbf_amd import pbf_torch  # noqa: F401  (the libtorch extension must load)
+    n = problem.scene.n
+    x0 = torch.from_numpy(problem.scene.x[:, :3].copy()).cuda().requires_grad_(True)
+    v0 = torch.from_numpy(problem.scene.v[:, :3].copy()).cuda().requires_grad_(True)
+    coef = torch.from_numpy(problem.roll.coefficients()).requires_grad_(True)
+    xT, vT = problem.roll(x0, v0, coef)
+    tgt = torch.from_numpy(problem.tgt).cuda()
+    loss = ((xT - tgt) ** 2).sum(1).mean() + 0.1 * (vT ** 2).sum(1).mean()
+    loss.backward()
+    assert np.array_equal(xT.detach().cpu().numpy(), problem.xT)
+    np.testing.assert_allclose(v0.grad.cpu().numpy(), problem.gv0[:, :3], rtol=1e-5, atol=1e-6 * np.abs(problem.gv0).max())
+    np.testing.assert_allclose(x0.grad.cpu().numpy(), problem.gx0[:, :3], rtol=1e-5, atol=1e-6 * np.abs(problem.gx0).max())
+    np.testing.assert_allclose(coef.grad.numpy(), problem.gc, rtol=1e-4)
+    assert coef.grad.device.type == "cpu" and n == x0.grad.shape[0]
+
+
+def test_dropped_friends_are_refused():
+    # 8x8x8 particles at 0.1h: every particle has far more than 50 friends in
+    # circle 0, so the friend lists are capped and not symmetric
+    x = pbf_amd.dam_break_block(8, 8, 8, 0.1, seed=0, spacing=0.1)
+    p = pbf_amd.physical_params(x.shape[0], 0.1, (0, 0, 0), (1.6, 1.6, 1.6))
+    sim = pbf_amd.Simulation(p, pbf_amd.default_options())
+    roll = Rollout(sim, 2, 2)
+    xT, vT = roll.forward_raw(_dev(x), _dev(np.zeros_like(x)))
+    with pytest.raises(pbf_amd.PbfError) as e:
+        roll.backward_raw(xT.clone(), None)
+    assert e.value.code == pbf_amd._lib.PBF_E_STATE
+    roll.close()
+    sim.close()

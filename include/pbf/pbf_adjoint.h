@@ -50,10 +50,11 @@ int pbf_get_scene_coefficients(pbf_ctx* ctx, uint32_t scene, float* out5);
 /* Device bytes of one recorded substep and of one checkpoint. */
 int pbf_adjoint_tape_bytes(pbf_ctx* ctx, uint64_t* per_substep, uint64_t* per_checkpoint);
 
-/* Checkpoint interval for `substeps` substeps under a device-memory budget:
- * the largest interval whose recorded substeps plus checkpoints fit, which
- * minimises the checkpoints kept (every interval is replayed once, so the
- * recompute cost does not depend on it; interval = substeps needs none). */
+/* Checkpoint interval for `substeps` substeps under a device-memory budget.
+ * The backward replays every interval but the last (which the forward
+ * records), so the plan takes the fewest intervals whose recorded substeps
+ * plus checkpoints fit, and among those the one with the longest last
+ * interval; interval = substeps replays nothing. */
 int pbf_adjoint_plan(pbf_ctx* ctx, uint32_t substeps, uint64_t budget_bytes, uint32_t* interval);
 
 /* A differentiable rollout of `substeps` substeps of ctx (which it then

@@ -115,6 +115,24 @@ __device__ __forceinline__ void block_sum(float (&v)[NV], float* out) {
     }
 }
 
+// Maximum over the workgroup (NaN propagates), written by thread 0 to *out.
+__device__ __forceinline__ void block_max(float v, float* out) {
+    __shared__ float s[kAdjThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float t = __shfl_xor(v, o, 64);
+        v = (t > v || t != t) ? t : v;
+    }
+    if (lane == 0) s[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float a = 0.0f;
+        for (int q = 0; q < kAdjThreads / 64; ++q) a = (s[q] > a || s[q] != s[q]) ? s[q] : a;
+        *out = a;
+    }
+}
+
 // Transpose of the walls (compute_delta.cl:1-42 as applied in apply_walls):
 // a coordinate a plane moved becomes lo + edge(F) or hi - edge(F), with
 // edge(F) = (3 + sum_c sin(5 F_c / h)) 0.03 h; the others pass through.
@@ -264,8 +282,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b1(const SceneConsts* __re
                                                         const float4* __restrict__ P, const float4* __restrict__ om,
                                                         const uint32_t* __restrict__ cnt,
                                                         const uint4* __restrict__ quads,
-                                                        const float4* __restrict__ gv, float4* __restrict__ z,
-                                                        float4* __restrict__ gom, float* __restrict__ part) {
+                                                        const float4* __restrict__ gv, const float* __restrict__ scale,
+                                                        float4* __restrict__ z, float4* __restrict__ gom,
+                                                        float* __restrict__ part) {
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint4 be = blocks[tile];
     const SceneConsts& c = scenes[be.x];
@@ -288,7 +307,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b1(const SceneConsts* __re
         const float l = sqrtf(dot(eta, eta));
         const V3 N = l > 0.0f ? eta * (1.0f / l) : eta;
         const V3 oi = v3(om[i]);
-        const V3 g = v3(gv[i]);
+        const V3 g = v3(gv[i]) * scale[be.x];
         ge[0] = c.dt * dot(cross(N, oi), g);
         const float u = c.vorticity * c.dt;
         const V3 gN = cross(oi, g) * u;
@@ -306,7 +325,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b2(
     const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ P,
     const float4* __restrict__ om, const uint32_t* __restrict__ cnt, const uint4* __restrict__ quads,
     const uint32_t* __restrict__ tcnt, const uint4* __restrict__ tquads, const float4* __restrict__ z,
-    float4* __restrict__ gom, float4* __restrict__ gx) {
+    const float* __restrict__ scale, float4* __restrict__ gom, float4* __restrict__ gx) {
     const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
     const SceneConsts& c = scenes[be.x];
     const uint32_t m = be.y + threadIdx.x;
@@ -339,7 +358,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b2(
         for_quads(m, t05(tcnt[m]), tquads, [&](const uint4& ii) { tr(ii.x); tr(ii.y); tr(ii.z); tr(ii.w); }, tr);
         if (wl > 0.0f) gom[m] = f4(v3(gom[m]) + omm * (gs / wl));
     }
-    gx[m] = f4(v3(gx[m]) + gP);
+    gx[m] = f4(v3(gx[m]) * scale[be.x] + gP);
 }
 
 // applyViscosity (apply_viscosity.cl:1-87, Jacobi) and updateVelocities
@@ -349,8 +368,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_xsph_b(
     const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ P,
     const float4* __restrict__ vu, const uint32_t* __restrict__ cnt, const uint4* __restrict__ quads,
     const uint32_t* __restrict__ tcnt, const uint4* __restrict__ tquads, const float4* __restrict__ gv,
-    const float4* __restrict__ gom, float4* __restrict__ gx, float4* __restrict__ gxs, float4* __restrict__ z,
-    const float4* __restrict__ fut, float* __restrict__ part) {
+    const float* __restrict__ scale, const float4* __restrict__ gom, float4* __restrict__ gx,
+    float4* __restrict__ gxs, float4* __restrict__ z, const float4* __restrict__ fut, float* __restrict__ part) {
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint4 be = blocks[tile];
     const SceneConsts& c = scenes[be.x];
@@ -359,7 +378,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_xsph_b(
     if (m < be.z) {
         const float4 p = P[m];
         const V3 vm = v3(vu[m]);
-        const V3 gvm = v3(gv[m]);
+        const float sc = scale[be.x];
+        const V3 gvm = v3(gv[m]) * sc;
         const float cp6 = c.viscosity * c.poly6;
         const V3 am = gvm * cp6;
         const V3 gam = v3(gom[m]) * -c.spiky;
@@ -384,7 +404,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_xsph_b(
                 Pair g;
                 if (pair_of(P[i], p, c, g)) {
                     const V3 u = vm - v3(vu[i]);
-                    const V3 ai = v3(gv[i]) * cp6;
+                    const V3 ai = (v3(gv[i]) * sc) * cp6;
                     const V3 gai = v3(gom[i]) * -c.spiky;
                     const float W = g.d * g.d * g.d;
                     gVu += ai * W + cross(psi(g), gai);
@@ -481,6 +501,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_lambda_b(
     const SceneConsts& c = scenes[be.x];
     const uint32_t m = be.y + threadIdx.x;
     float gg[3] = {0.0f, 0.0f, 0.0f};
+    float mx = 0.0f;
     if (m < be.z) {
         const float4 p = P[m];
         const float4 qm = z2[m];
@@ -513,33 +534,92 @@ __global__ __launch_bounds__(kAdjThreads) void k_lambda_b(
             z[m] = f4(gF * (-c.spiky / c.rest_density));
         } else {
             const uint32_t pp = perm[m];
+            const V3 ox = v3(gxs[m]) + gPk;
+            const V3 ov = gPk * c.dt;
             if (pp < n) {
-                gx_out[pp] = f4(v3(gxs[m]) + gPk);
-                gv_out[pp] = f4(gPk * c.dt);
+                gx_out[pp] = f4(ox);
+                gv_out[pp] = f4(ov);
             }
+            mx = fmaxf(fmaxf(fmaxf(fabsf(ox.x), fabsf(ox.y)), fmaxf(fabsf(ox.z), fabsf(ov.x))),
+                       fmaxf(fabsf(ov.y), fabsf(ov.z)));
             const float d2 = c.dt * c.dt;
             gg[0] = gPk.x * d2;
             gg[1] = gPk.y * d2;
             gg[2] = gPk.z * d2;
         }
     }
-    if (FIRST) block_sum<3>(gg, part + (size_t)tile * kParts + 2);
+    if (FIRST) {
+        block_sum<3>(gg, part + (size_t)tile * kParts + 2);
+        block_max(mx, part + (size_t)tile * kParts + 5);
+    }
 }
 
 // Per-scene sums of the per-tile partials (tiles of a scene are contiguous),
-// added to the running totals.  One wave per scene.
+// added to the running totals in true units, and the power-of-two rescale of
+// the scene's adjoint state for the next substep.  Sensitivities of these
+// flows grow geometrically with the horizon; the state (dL/dx, dL/dv) is kept
+// as true * 2^expo[s] with its largest component in [2^-16, 2^16], so that
+// no intermediate product overflows binary32.  Scaling by a power of two is
+// exact: the results equal the unscaled arithmetic wherever that stays
+// finite.  One wave per scene.
 __global__ __launch_bounds__(64) void k_param_reduce(const uint2* __restrict__ tile_range,
-                                                    const float* __restrict__ part, float* __restrict__ gparam) {
+                                                    const float* __restrict__ part, float* __restrict__ gparam,
+                                                    float* __restrict__ scale, int* __restrict__ expo) {
     const uint32_t s = blockIdx.x;
     const uint2 tr = tile_range[s];
     const int lane = threadIdx.x;
+    const int E = expo[s];
     for (int k = 0; k < 5; ++k) {
         float a = 0.0f;
         for (uint32_t t = tr.x + lane; t < tr.y; t += 64) a += part[(size_t)t * kParts + k];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-        if (lane == 0) gparam[s * 5 + k] += a;
+        if (lane == 0) gparam[s * 5 + k] += ldexpf(a, -E);
     }
+    float mx = 0.0f;
+    for (uint32_t t = tr.x + lane; t < tr.y; t += 64) {
+        const float v = part[(size_t)t * kParts + 5];
+        mx = (v > mx || v != v) ? v : mx;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float t = __shfl_xor(mx, o, 64);
+        mx = (t > mx || t != t) ? t : mx;
+    }
+    if (lane == 0) {
+        int k = 0;
+        if (mx > 0.0f && mx <= 3.0e38f) {
+            int e;
+            (void)frexpf(mx, &e);   // mx = f 2^e, f in [0.5, 1)
+            if (e > 16 || e < -16) k = -e;
+        }
+        scale[s] = ldexpf(1.0f, k);
+        expo[s] = E + k;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_scale_init(float* __restrict__ scale, int* __restrict__ expo, uint32_t S) {
+    const uint32_t s = blockIdx.x * 64 + threadIdx.x;
+    if (s < S) {
+        scale[s] = 1.0f;
+        expo[s] = 0;
+    }
+}
+
+// Gradients in true units (identity order: the rollout's first substep).
+// The state is still in the units of the last substep: the rescale chosen
+// after it (scale = 2^k, already counted in expo) has not been applied.
+__global__ __launch_bounds__(kAdjThreads) void k_unscale(const uint4* __restrict__ blocks,
+                                                        const float* __restrict__ scale, const int* __restrict__ expo,
+                                                        const float4* __restrict__ gx, const float4* __restrict__ gv,
+                                                        float4* __restrict__ gx0, float4* __restrict__ gv0) {
+    const uint4 be = blocks[blockIdx.x];
+    const uint32_t i = be.y + threadIdx.x;
+    if (i >= be.z) return;
+    const int e = ilogbf(scale[be.x]) - expo[be.x];
+    const float4 a = gx[i], b = gv[i];
+    if (gx0) gx0[i] = make_float4(ldexpf(a.x, e), ldexpf(a.y, e), ldexpf(a.z, e), 0.0f);
+    if (gv0) gv0[i] = make_float4(ldexpf(b.x, e), ldexpf(b.y, e), ldexpf(b.z, e), 0.0f);
 }
 
 // One recorded substep (device arrays).
@@ -577,6 +657,8 @@ struct pbf_adjoint {
     float* zb = nullptr;
     float* part = nullptr;
     float* gparam = nullptr;
+    float* scale = nullptr;                // per scene: rescale of the adjoint state for the next substep
+    int* expo = nullptr;                   // per scene: state = true * 2^expo
     uint2* tile_range = nullptr;
     unsigned long long* notes = nullptr;   // C + 1 snapshots of the dropped-friends counter
     uint32_t* flags = nullptr;
@@ -718,12 +800,12 @@ int backward_substep(pbf_adjoint* a, const TapeSlot& t) {
     const dim3 grid(bt.count), block(kAdjThreads);
     hipStream_t st = ctx->stream;
     const float4* PK = t.P + (size_t)K * n;
-    hipLaunchKernelGGL(k_vort_b1, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.om, t.cnt, t.quads, a->gv, a->z,
-                       a->gom, a->part);
+    hipLaunchKernelGGL(k_vort_b1, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.om, t.cnt, t.quads, a->gv, a->scale,
+                       a->z, a->gom, a->part);
     hipLaunchKernelGGL(k_vort_b2, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.om, t.cnt, t.quads, t.tcnt, t.tquads,
-                       a->z, a->gom, a->gx);
+                       a->z, a->scale, a->gom, a->gx);
     hipLaunchKernelGGL(k_xsph_b, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.vu, t.cnt, t.quads, t.tcnt, t.tquads,
-                       a->gv, a->gom, a->gx, a->gxs, a->z, t.fut + (size_t)(K - 1) * n, a->part);
+                       a->gv, a->scale, a->gom, a->gx, a->gxs, a->z, t.fut + (size_t)(K - 1) * n, a->part);
     for (int k = K - 1; k >= 0; --k) {
         const float4* Pk = t.P + (size_t)k * n;
         hipLaunchKernelGGL(k_delta_b, grid, block, 0, st, ctx->sc_d, bt.d, Pk, t.aux + (size_t)k * n, t.cnt,
@@ -738,7 +820,8 @@ int backward_substep(pbf_adjoint* a, const TapeSlot& t) {
                                a->part, ctx->n);
         }
     }
-    hipLaunchKernelGGL(k_param_reduce, dim3(ctx->S), dim3(64), 0, st, a->tile_range, a->part, a->gparam);
+    hipLaunchKernelGGL(k_param_reduce, dim3(ctx->S), dim3(64), 0, st, a->tile_range, a->part, a->gparam, a->scale,
+                       a->expo);
     HIP_TRY(ctx, hipGetLastError());
     std::swap(a->gx, a->gx2);
     std::swap(a->gv, a->gv2);
@@ -797,15 +880,24 @@ int pbf_adjoint_plan(pbf_ctx* ctx, uint32_t substeps, uint64_t budget_bytes, uin
     if (!ctx || !interval || substeps == 0) return PBF_E_INVALID;
     const uint64_t slot = pbf::slot_bytes(ctx->n, ctx->iters), ck = 32ull * ctx->n;
     const uint64_t fixed = 4ull * 2 * ctx->n + 16ull * 8 * ctx->n + 4ull * ctx->n;   // ids, gradients, coefficients
+    // Largest interval that fits: the fewest intervals.  The forward records
+    // the last interval itself, so the backward replays all substeps but
+    // those of the last one; among the intervals giving the fewest intervals,
+    // the smallest makes the last one longest.
+    uint32_t cmax = 0;
     for (uint32_t C = substeps; C >= 1; --C) {
         const uint64_t segs = (substeps + C - 1) / C;
         if (fixed + C * slot + segs * ck <= budget_bytes) {
-            *interval = C;
-            return PBF_OK;
+            cmax = C;
+            break;
         }
     }
-    return pbf::fail(ctx, PBF_E_NOMEM, "one recorded substep (%llu B) does not fit in %llu B",
-                     (unsigned long long)slot, (unsigned long long)budget_bytes);
+    if (cmax == 0)
+        return pbf::fail(ctx, PBF_E_NOMEM, "one recorded substep (%llu B) does not fit in %llu B",
+                         (unsigned long long)slot, (unsigned long long)budget_bytes);
+    const uint32_t segs = (substeps + cmax - 1) / cmax;
+    *interval = (substeps + segs - 1) / segs;
+    return PBF_OK;
 }
 
 int pbf_adjoint_create(pbf_ctx* ctx, uint32_t substeps, uint32_t interval, pbf_adjoint** out) {
@@ -862,6 +954,8 @@ int pbf_adjoint_create(pbf_ctx* ctx, uint32_t substeps, uint32_t interval, pbf_a
     const pbf::BlockTable& bt = ctx->bt[2];
     ADJ_TRY(pbf::aalloc(a, &a->part, (size_t)bt.count * pbf::kParts));
     ADJ_TRY(pbf::aalloc(a, &a->gparam, (size_t)ctx->S * 5));
+    ADJ_TRY(pbf::aalloc(a, &a->scale, ctx->S));
+    ADJ_TRY(pbf::aalloc(a, &a->expo, ctx->S));
     ADJ_TRY(pbf::aalloc(a, &a->tile_range, ctx->S));
     ADJ_TRY(pbf::aalloc(a, &a->notes, (size_t)a->C + 1));
     ADJ_TRY(pbf::aalloc(a, &a->flags, 1));
@@ -952,6 +1046,8 @@ int pbf_adjoint_backward(pbf_adjoint* a, const float* gxT, const float* gvT, flo
                        ctx->n);
     hipLaunchKernelGGL(pbf::k_fill, dim3((ctx->S * 5 + 255) / 256), dim3(256), 0, ctx->stream, a->gparam,
                        (size_t)ctx->S * 5);
+    hipLaunchKernelGGL(pbf::k_scale_init, dim3((ctx->S + 63) / 64), dim3(64), 0, ctx->stream, a->scale, a->expo,
+                       ctx->S);
     HIP_TRY(ctx, hipGetLastError());
     for (int g = (int)a->segs - 1; g >= 0; --g) {
         if (a->recorded != g) {
@@ -970,8 +1066,12 @@ int pbf_adjoint_backward(pbf_adjoint* a, const float* gxT, const float* gvT, flo
             if (rc) return rc;
         }
     }
-    if (gx0) HIP_TRY(ctx, hipMemcpyAsync(gx0, a->gx, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    if (gv0) HIP_TRY(ctx, hipMemcpyAsync(gv0, a->gv, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    if (gx0 || gv0) {
+        const pbf::BlockTable& bt = ctx->bt[2];
+        hipLaunchKernelGGL(pbf::k_unscale, dim3(bt.count), dim3(pbf::kAdjThreads), 0, ctx->stream, bt.d, a->scale,
+                           a->expo, a->gx, a->gv, reinterpret_cast<float4*>(gx0), reinterpret_cast<float4*>(gv0));
+        HIP_TRY(ctx, hipGetLastError());
+    }
     if (gcoef)
         HIP_TRY(ctx, hipMemcpyAsync(gcoef, a->gparam, 4 * 5 * (size_t)ctx->S, hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

@@ -63,15 +63,19 @@ class Simulation:
 
     def Step(self):
         """Simulation::Step (src/Simulation.cpp:554-621), asynchronous."""
+        self._ensure()
         check(lib().pbf_step(self._ctx, 1 if self.bPauseSim else 0), self._ctx)
 
     def run(self, steps: int):
+        self._ensure()
         check(lib().pbf_run(self._ctx, steps, 1 if self.bPauseSim else 0), self._ctx)
 
     def sync(self):
+        self._ensure()
         check(lib().pbf_sync(self._ctx), self._ctx)
 
     def get_state(self) -> dict:
+        self._ensure()
         n = self.n
         out = {
             "x": np.empty((n, 4), np.float32),
@@ -89,6 +93,7 @@ class Simulation:
         return int(self.params.gridBufSize)
 
     def get_structure(self, friends: bool = True) -> dict:
+        self._ensure()
         n = self.n
         out = {
             "keys": np.empty(n, np.uint32),
@@ -103,15 +108,18 @@ class Simulation:
         return out
 
     def device_buffers(self) -> PbfDevBufs:
+        self._ensure()
         b = PbfDevBufs()
         check(lib().pbf_device_buffers(self._ctx, ctypes.byref(b)), self._ctx)
         return b
 
     # -- OCLPerfMon-style telemetry -------------------------------------------
     def set_profiling(self, enable: bool):
+        self._ensure()
         check(lib().pbf_set_profiling(self._ctx, 1 if enable else 0), self._ctx)
 
     def timings(self) -> dict:
+        self._ensure()
         cnt = ctypes.c_int()
         check(lib().pbf_timings(self._ctx, None, 0, ctypes.byref(cnt)), self._ctx)
         arr = (PbfTiming * max(cnt.value, 1))()
@@ -122,27 +130,33 @@ class Simulation:
         }
 
     def reset_timings(self):
+        self._ensure()
         check(lib().pbf_reset_timings(self._ctx), self._ctx)
 
     def set_graph(self, enable: bool):
         """Replay substeps as captured HIP graphs."""
+        self._ensure()
         check(lib().pbf_set_graph(self._ctx, 1 if enable else 0), self._ctx)
 
     def set_workgroup_sizes(self, sizes: dict):
+        self._ensure()
         arr = (ctypes.c_uint32 * len(FAMILIES))(*[int(sizes[f]) for f in FAMILIES])
         check(lib().pbf_set_workgroup_sizes(self._ctx, arr), self._ctx)
 
     def set_friends_mode(self, staged: bool):
         """Friends build variant: LDS-staged neighbourhood or global reads."""
+        self._ensure()
         check(lib().pbf_set_friends_mode(self._ctx, 1 if staged else 0), self._ctx)
 
     def autotune(self, steps: int = 3) -> dict:
         """Times every workgroup size per kernel family; keeps the fastest."""
+        self._ensure()
         arr = (ctypes.c_uint32 * len(FAMILIES))()
         check(lib().pbf_autotune(self._ctx, steps, arr), self._ctx)
         return dict(zip(FAMILIES, (int(a) for a in arr)))
 
     def stats(self) -> dict:
+        self._ensure()
         st = PbfStats()
         check(lib().pbf_stats(self._ctx, ctypes.byref(st)), self._ctx)
         out = {name: getattr(st, name) for name, _ in PbfStats._fields_ if name not in ("workgroup_size", "reserved")}
@@ -150,6 +164,7 @@ class Simulation:
         return out
 
     def error_flags(self, reset: bool = False) -> tuple[int, int]:
+        self._ensure()
         f = ctypes.c_uint32()
         d = ctypes.c_uint64()
         check(lib().pbf_error_flags(self._ctx, ctypes.byref(f), ctypes.byref(d), 1 if reset else 0), self._ctx)

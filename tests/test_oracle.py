@@ -351,3 +351,114 @@ def test_dam_break_stays_in_box(oracle):
     assert np.all(x >= 0.0) and np.all(x <= 1.6)
     speed = np.linalg.norm(st["v"][:, :3], axis=1)
     assert speed.max() < 20.0
+
+
+# --- stages pinned by kinematics and two-particle closed forms ----------------
+
+def _cfg(n=2, h=0.1, dt=1.0 / 128.0, gravity=(0.0, -8.0, 0.0), visc=0.0, vort=0.0):
+    p = pbf_amd.physical_params(n, h, (0, 0, 0), (1, 1, 1))
+    c = po.cfg_from_params(p, pbf_amd.default_options(walls=False))
+    c.dt = dt
+    c.gravity[0], c.gravity[1], c.gravity[2] = gravity
+    c.viscosity = visc
+    c.vorticity = vort
+    return c
+
+
+def test_predict_is_semi_implicit_euler(oracle):
+    """predict_positions.cl:12-17: v += dt g (unless paused), x* = x + dt v on
+    the float4; dyadic inputs make every result exact."""
+    c = _cfg(n=1)
+    x = np.array([[1.0, 2.0, 3.0, 1.0]], np.float32)
+    v = np.array([[0.5, 0.25, -1.0, 2.0]], np.float32)
+    xs = np.zeros_like(x)
+    vv = v.copy()
+    po.lib().pbfo_predict(ctypes.byref(c), 0, po.ptr(x), po.ptr(vv), po.ptr(xs))
+    dt = 1.0 / 128.0
+    assert vv.tolist() == [[0.5, 0.25 - 8.0 * dt, -1.0, 2.0]]
+    # x* = x + dt v + dt^2 g
+    assert xs.tolist() == [[1.0 + dt * 0.5, 2.0 + dt * 0.25 - 8.0 * dt * dt, 3.0 - dt, 1.0 + 2.0 * dt]]
+    vv = v.copy()
+    po.lib().pbfo_predict(ctypes.byref(c), 1, po.ptr(x), po.ptr(vv), po.ptr(xs))
+    assert vv.tolist() == v.tolist()
+    assert xs.tolist() == [[1.0 + dt * 0.5, 2.0 + dt * 0.25, 3.0 - dt, 1.0 + 2.0 * dt]]
+
+
+def test_update_velocities_known_answer(oracle):
+    """update_velocities.cl:13-15: v = (x* - x)/dt, x = x*, x.w = |v| (3-4-5)."""
+    c = _cfg(n=1)
+    x = np.array([[0.25, 0.5, 0.75, 9.0]], np.float32)
+    xs = np.array([[0.25 + 3.0 / 128, 0.5 + 4.0 / 128, 0.75, 7.0]], np.float32)
+    v = np.full((1, 4), -1.0, np.float32)
+    po.lib().pbfo_update_velocities(ctypes.byref(c), po.ptr(x), po.ptr(xs), po.ptr(v))
+    assert v[0, :3].tolist() == [3.0, 4.0, 0.0] and v[0, 3] == -1.0
+    assert x[0].tolist() == [0.25 + 3.0 / 128, 0.5 + 4.0 / 128, 0.75, 5.0]
+
+
+def test_pack_and_update_predicted(oracle):
+    """pack_data.cl: (src.xyz, s); update_predicted.cl: x* + delta on xyz."""
+    src = np.array([[1, 2, 3, 4], [5, 6, 7, 8]], np.float32)
+    w = np.array([0.5, -0.25], np.float32)
+    dst = np.zeros_like(src)
+    po.lib().pbfo_pack(2, po.ptr(src), po.ptr(w), po.ptr(dst))
+    assert dst.tolist() == [[1, 2, 3, 0.5], [5, 6, 7, -0.25]]
+    delta = np.array([[0.5, 0.25, -1, 100], [0, 0, 0.125, 100]], np.float32)
+    out = np.zeros_like(src)
+    po.lib().pbfo_update_predicted(2, po.ptr(dst), po.ptr(delta), po.ptr(out))
+    assert out.tolist() == [[1.5, 2.25, 2, 0.5], [5, 6, 7.125, -0.25]]
+
+
+def _pair(d, h=0.1):
+    xs = np.zeros((2, 4), np.float32)
+    xs[0, :3] = (0.5, 0.5, 0.5)
+    xs[1, :3] = (0.5 + d, 0.5, 0.5)
+    xs[:, 3] = 1.0                      # densities (the rho_j mask passes)
+    fr = np.zeros(2 * 255, np.uint32)
+    fr[0:2] = 1                         # one friend each, circle 0
+    fr[5 * 2 + 0] = 1
+    fr[5 * 2 + 1] = 0
+    return xs, fr
+
+
+def test_viscosity_two_particle_closed_form(oracle):
+    """apply_viscosity.cl: v_i += c POLY6 (v_j - v_i)(h^2 - r^2)^3 and
+    omega_i = -SPIKY (v_j - v_i) x r^(h - r)^2, for one pair; no
+    contribution from a neighbour whose density is 0."""
+    h, d, cvisc = 0.1, 0.04, 0.3
+    c = _cfg(visc=cvisc)
+    xs, fr = _pair(d, h)
+    v = np.array([[0.1, 0.2, 0.3, 0], [1.0, -2.0, 0.5, 0]], np.float32)
+    vo = np.zeros_like(v)
+    om = np.zeros_like(v)
+    po.lib().pbfo_viscosity(ctypes.byref(c), po.ptr(xs), po.ptr(v), po.ptr(fr), po.ptr(vo), po.ptr(om))
+    poly6, spiky = oracle.kernel_factors(h)
+    u = (v[1, :3] - v[0, :3]).astype(np.float64)
+    W = (h * h - d * d) ** 3
+    np.testing.assert_allclose(vo[0, :3], v[0, :3] + cvisc * poly6 * W * u, rtol=1e-5)
+    np.testing.assert_allclose(vo[1, :3], v[1, :3] - cvisc * poly6 * W * u, rtol=1e-5)
+    s = (h - d) ** 2                     # |psi|; psi_0 = (-s, 0, 0), psi_1 = (s, 0, 0)
+    np.testing.assert_allclose(om[0, :3], -spiky * np.array([0.0, -u[2] * s, u[1] * s]), rtol=1e-5)
+    np.testing.assert_allclose(om[1, :3], -spiky * np.array([0.0, -u[2] * s, u[1] * s]), rtol=1e-5)
+    xs[1, 3] = 0.0                       # neighbour with zero density: skipped
+    po.lib().pbfo_viscosity(ctypes.byref(c), po.ptr(xs), po.ptr(v), po.ptr(fr), po.ptr(vo), po.ptr(om))
+    assert vo[0, :3].tolist() == v[0, :3].tolist() and om[0, :3].tolist() == [0.0, 0.0, 0.0]
+
+
+def test_vorticity_two_particle_closed_form(oracle):
+    """apply_vorticity.cl: N = eta/|eta| with eta = -SPIKY sum |w_j| grad, so
+    for one pair N points from the neighbour to the particle; v_i += dt eps
+    N x w_i.  With |w_j| = 0, eta = 0 and v is unchanged."""
+    h, d, eps, dt = 0.1, 0.05, 0.5, 1.0 / 128.0
+    c = _cfg(vort=eps, dt=dt)
+    xs, fr = _pair(d, h)
+    om = np.array([[0, 0, 1, 0], [0, 0, 2, 0]], np.float32)
+    v = np.zeros((2, 4), np.float32)
+    po.lib().pbfo_vorticity(ctypes.byref(c), po.ptr(xs), po.ptr(v), po.ptr(om), po.ptr(fr))
+    # particle 0: N = (1, 0, 0) (spiky > 0, r = p0 - p1 = -d x); N x (0,0,1) = (0,-1,0)
+    np.testing.assert_allclose(v[0, :3], [0.0, -dt * eps, 0.0], rtol=1e-6, atol=1e-12)
+    # particle 1: N = (-1, 0, 0); N x (0,0,2) = (0, 2, 0)
+    np.testing.assert_allclose(v[1, :3], [0.0, 2.0 * dt * eps, 0.0], rtol=1e-6, atol=1e-12)
+    om[1, 2] = 0.0
+    v[:] = 0.0
+    po.lib().pbfo_vorticity(ctypes.byref(c), po.ptr(xs), po.ptr(v), po.ptr(om), po.ptr(fr))
+    assert v[0, :3].tolist() == [0.0, 0.0, 0.0]

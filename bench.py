@@ -37,6 +37,13 @@ STAGE_BYTES = {
 }
 
 
+# HBM bytes per launch measured with rocprofv3 PMC counters on the
+# 1,048,576-particle dam break (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a
+# plain 256 MiB copy gives the two counters), averaged over the K launches of
+# a substep; profiles/pmc_traffic_dam1m.md holds the counts and commands.
+PMC_TRAFFIC_BYTES_PER_LAUNCH = {"computeDelta": 185.9e6}
+
+
 def kernel_bytes_per_particle(name: str, iters: int, passes: int) -> float | None:
     base = name.rsplit("_", 1)[0] if name.rsplit("_", 1)[-1].isdigit() else name
     if base == "predictPositions":
@@ -140,6 +147,10 @@ def batched_gain(steps: int, warmup: int, seed: int, tune: bool) -> dict:
         "gain": t_loop / t_batch,
         "workgroup_sizes": sizes,
     }
+
+
+def _progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def _dev4(a):
@@ -329,6 +340,7 @@ def main() -> int:
     sim.close()
 
     value = n * args.steps / el
+    _progress(f"forward: {value:.4g} particle-substeps/s")
     passes = 0
     while (1 << (8 * passes)) < int(scene.params.gridBufSize):
         passes += 1
@@ -381,7 +393,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": PMC_TRAFFIC_BYTES_PER_LAUNCH.get(dom) if n == 1048576 else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
             "top_kernel_overall": top,
@@ -393,12 +405,16 @@ def main() -> int:
     if not args.no_batched:
         out["batched"] = batched_gain(args.batched_steps, 5, args.seed, not args.no_autotune)
         out["batched_gain"] = out["batched"]["gain"]
+        _progress(f"batched gain {out['batched_gain']:.3f}")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_ratio(args.seed)
-        out["batched_adjoint"] = batched_adjoint(args.seed)
         out["adjoint_ratio"] = out["adjoint"]["ratio"]
+        _progress(f"adjoint/forward {out['adjoint_ratio']:.3f}")
+        out["batched_adjoint"] = batched_adjoint(args.seed)
+        _progress(f"batched adjoint/forward {out['batched_adjoint']['ratio']:.3f}")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget)
+        _progress("cpu baseline done")
     print(json.dumps(out))
     return 0
 

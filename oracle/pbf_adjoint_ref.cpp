@@ -199,7 +199,8 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                         const T rl = t_sqrt(r2);
                         const T hr = h - rl;
                         const T sp = spiky * hr * hr;
-                        const T gg[3] = {sp * r[0] / rl, sp * r[1] / rl, sp * r[2] / rl};
+                        const T inv = one / rl;
+                        const T gg[3] = {sp * r[0] * inv, sp * r[1] * inv, sp * r[2] * inv};
                         const T d = h2 - r2;
                         dens += d * d * d;
                         gsum += gg[0] * gg[0] + gg[1] * gg[1] + gg[2] * gg[2];
@@ -225,7 +226,8 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                     if (val(r2) < val(h2)) {
                         const T rl = t_sqrt(r2);
                         const T hr = h - rl;
-                        const T gg[3] = {r[0] / rl * hr * hr, r[1] / rl * hr * hr, r[2] / rl * hr * hr};
+                        const T inv = one / rl;
+                        const T gg[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
                         const T d = h2 - r2;
                         const T p6 = d * d * d;
                         const T ratio = p6 / poly6_q;
@@ -271,7 +273,8 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                     const T d = h2 - r2;
                     const T rl = t_sqrt(r2);
                     const T hr = h - rl;
-                    const T gg[3] = {r[0] / rl * hr * hr, r[1] / rl * hr * hr, r[2] / rl * hr * hr};
+                    const T inv = one / rl;
+                        const T gg[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
                     o[0] += u[1] * gg[2] - u[2] * gg[1];
                     o[1] += u[2] * gg[0] - u[0] * gg[2];
                     o[2] += u[0] * gg[1] - u[1] * gg[0];
@@ -298,7 +301,8 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                 if (val(r2) < val(h2) && fabs(val(rho[j])) > (double)1e-8f) {
                     const T rl = t_sqrt(r2);
                     const T hr = h - rl;
-                    const T gg[3] = {r[0] / rl * hr * hr, r[1] / rl * hr * hr, r[2] / rl * hr * hr};
+                    const T inv = one / rl;
+                        const T gg[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
                     const T* oj = om[j].a;
                     const T wl = t_sqrt(oj[0] * oj[0] + oj[1] * oj[1] + oj[2] * oj[2]);
                     eta[0] += wl * gg[0]; eta[1] += wl * gg[1]; eta[2] += wl * gg[2];

@@ -256,7 +256,8 @@ void pbfo_scaling(const pbfo_cfg *c, const float *xs, const uint32_t *fr, float 
                     const float rl = sqrtf(r2);
                     const float hr = h - rl;
                     const float s = spiky * hr * hr;
-                    const float g[3] = {s * r[0] / rl, s * r[1] / rl, s * r[2] / rl};
+                    const float inv = 1.0f / rl;   /* r / |r| as r * (1/|r|) */
+                    const float g[3] = {s * r[0] * inv, s * r[1] * inv, s * r[2] * inv};
                     const float d = h2 - r2;
                     dens += d * d * d;
                     gsum += dot3(g, g);
@@ -349,7 +350,8 @@ void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const u
                 if (r2 < h2) {
                     const float rl = sqrtf(r2);
                     const float hr = h - rl;
-                    const float g[3] = {r[0] / rl * hr * hr, r[1] / rl * hr * hr, r[2] / rl * hr * hr};
+                    const float inv = 1.0f / rl;   /* r / |r| as r * (1/|r|) */
+                    const float g[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
                     const float d = h2 - r2;
                     const float p6 = d * d * d;
                     const float ratio = p6 / poly6_q;
@@ -444,7 +446,8 @@ void pbfo_viscosity(const pbfo_cfg *c, const float *xs, const float *v_in, const
                     const float d = h2 - r2;
                     const float rl = sqrtf(r2);
                     const float hr = h - rl;
-                    const float g[3] = {r[0] / rl * hr * hr, r[1] / rl * hr * hr, r[2] / rl * hr * hr};
+                    const float inv = 1.0f / rl;   /* r / |r| as r * (1/|r|) */
+                    const float g[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
                     float cr[3];
                     cross3(vij, g, cr);
                     om[0] += cr[0]; om[1] += cr[1]; om[2] += cr[2];
@@ -487,7 +490,8 @@ void pbfo_vorticity(const pbfo_cfg *c, const float *xs, float *v, const float *o
                 if (r2 < h2 && fabsf(pj[3]) > 1e-8f) {
                     const float rl = sqrtf(r2);
                     const float hr = h - rl;
-                    const float g[3] = {r[0] / rl * hr * hr, r[1] / rl * hr * hr, r[2] / rl * hr * hr};
+                    const float inv = 1.0f / rl;   /* r / |r| as r * (1/|r|) */
+                    const float g[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
                     const float *oj = F4(omega, j);
                     const float wl = sqrtf(dot3(oj, oj));
                     eta[0] += wl * g[0]; eta[1] += wl * g[1]; eta[2] += wl * g[2];

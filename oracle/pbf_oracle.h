@@ -11,7 +11,11 @@
  * src/Simulation.cpp:554-621, with IEEE single precision, no FMA contraction
  * (built with -ffp-contract=off) and the operation order of the reference's
  * expressions.  Where the reference relies on -cl-fast-relaxed-math or has
- * undefined behaviour, the choice made here is stated at the function.
+ * undefined behaviour, the choice made here is stated at the function; one
+ * applies to every friend sweep: the vector r / |r| (compute_scaling.cl:83,
+ * compute_delta.cl:144, apply_viscosity.cl:65, apply_vorticity.cl) is
+ * evaluated as r * (1/|r|), one correctly rounded reciprocal per pair, which
+ * the relaxed-math build of the reference also permits.
  *
  * Data layout (all little-endian, particle-major):
  *   float4 arrays   : float[4*n]   (x, y, z, w)

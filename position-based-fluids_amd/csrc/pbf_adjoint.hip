@@ -176,9 +176,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_transpose(const SceneConsts* __
     for (int pass = 0; pass < 2; ++pass) {
         for (int q = 0; q < kCircles; ++q) {
             const int cq = circle_count(pm, q);
-            const uint32_t* seg = nb_list + friend_seg(m, q);
             for (int f = 0; f < cq; ++f) {
-                const uint32_t i = seg[f];
+                const uint32_t i = nb_list[friend_slot(m, q, f)];
                 const uint32_t pi = cnt[i];
                 int dummy;
                 const int l05 = circles_processed<1, 2>(pi, &dummy);

@@ -250,7 +250,7 @@ int create_impl(const pbf_params* params, const pbf_options* opts, uint32_t S, p
     CREATE_TRY(dalloc(&ctx->perm, n));
     CREATE_TRY(dalloc(&ctx->cells, 2 * (size_t)ctx->total_cells));
     CREATE_TRY(dalloc(&ctx->nb_cnt, n));
-    CREATE_TRY(dalloc(&ctx->nb_list, n * pbf::kCircles * (size_t)pbf::kSeg));
+    CREATE_TRY(dalloc(&ctx->nb_list, pbf::friend_slots(n)));
     CREATE_TRY(dalloc(&ctx->nb_quads, ((n + 63) / 64) * 64 * (size_t)pbf::kQuads));
     CREATE_TRY(dalloc(&ctx->err, pbf::kErrWords));
     CREATE_TRY(dalloc(&ctx->sc_d, S));
@@ -556,19 +556,19 @@ int pbf_get_structure(pbf_ctx* ctx, uint32_t* keys_sorted, uint32_t* perm, uint3
     if (cells) HIP_TRY(ctx, hipMemcpy(cells, ctx->cells, 8 * (size_t)ctx->total_cells, hipMemcpyDeviceToHost));
     if (friends && n) {
         // counts[5][n] unpacked from the 6-bit fields, then indices[5][50][n]
-        // regrouped from the particle-major circle segments (zero past each count)
+        // regrouped from the blocked circle slots (zero past each count)
         std::vector<uint32_t> packed(n);
         HIP_TRY(ctx, hipMemcpy(packed.data(), ctx->nb_cnt, 4 * n, hipMemcpyDeviceToHost));
-        std::vector<uint32_t> segs(n * pbf::kCircles * (size_t)pbf::kSeg);
-        HIP_TRY(ctx, hipMemcpy(segs.data(), ctx->nb_list, 4 * segs.size(), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> slots(pbf::friend_slots(n));
+        HIP_TRY(ctx, hipMemcpy(slots.data(), ctx->nb_list, 4 * slots.size(), hipMemcpyDeviceToHost));
         uint32_t* idx = friends + (size_t)pbf::kCircles * n;
         memset(idx, 0, 4 * n * pbf::kCircles * pbf::kPerCircle);
         for (size_t i = 0; i < n; ++i)
             for (int q = 0; q < pbf::kCircles; ++q) {
                 const uint32_t cnt = (packed[i] >> (6 * q)) & 63u;
                 friends[(size_t)q * n + i] = cnt;
-                const uint32_t* seg = segs.data() + (i * pbf::kCircles + q) * pbf::kSeg;
-                for (uint32_t f = 0; f < cnt; ++f) idx[((size_t)q * pbf::kPerCircle + f) * n + i] = seg[f];
+                for (uint32_t f = 0; f < cnt; ++f)
+                    idx[((size_t)q * pbf::kPerCircle + f) * n + i] = slots[pbf::friend_slot((uint32_t)i, q, (int)f)];
             }
     }
     return PBF_OK;

@@ -99,17 +99,19 @@ __device__ __forceinline__ int circles_processed(uint32_t packed, int* m) {
     return L;
 }
 
-// Friends lists, particle-major: particle i owns five 16-byte aligned
-// segments of kSeg slots, segment c holding its circle-c friends in the scan
-// order of the reference.  This is the reference's layout (counts[5][n],
-// indices[5][50][n], build_friends_list.cl:1-21) with the particle index
-// moved outermost: a thread appends to its own segments as it finds friends
-// (each cache line is filled by one thread) and a sweep reads four friends
-// per 16-byte load.
-constexpr int kSeg = 52;
-__device__ __forceinline__ size_t friend_seg(uint32_t i, int circle) {
-    return ((size_t)i * kCircles + (size_t)circle) * kSeg;
+// Friends lists: particle i owns, for each of its five circles, kSeg slots
+// holding its circle-c friends in the reference's scan order -- the
+// reference's layout (counts[5][n], indices[5][50][n],
+// build_friends_list.cl:1-21) blocked per 64 consecutive particles: slot f of
+// circle c of the block's 64 particles is one contiguous 256-byte row
+// [(i>>6)][c][f][lane], so the appends of a wave's lanes land in the same few
+// rows (a small write footprint that stays in L2) and a wave's reads of one
+// slot are coalesced.
+constexpr int kSeg = kPerCircle;
+__device__ __host__ __forceinline__ size_t friend_slot(uint32_t i, int circle, int f) {
+    return (((size_t)(i >> 6) * kCircles + (size_t)circle) * kSeg + (size_t)f) * 64 + (i & 63u);
 }
+__device__ __host__ __forceinline__ size_t friend_slots(size_t n) { return ((n + 63) / 64) * 64 * kCircles * kSeg; }
 
 // Sweep lists: the friends a sweep processes (the 0.6-threshold prefix, in
 // the reference's order) regrouped per block of 64 consecutive particles in
@@ -143,6 +145,47 @@ __device__ __forceinline__ void for_friends(uint32_t i, uint32_t packed, const u
     int m;
     (void)circles_processed<L_NUM, L_DEN>(packed, &m);
     for_quads(i, m, quads, fn4, fn1);
+}
+
+// The same walk in two phases per quad, for the sweeps: load(j) issues the
+// gathers of the four friends, the next quad's indices are fetched while they
+// are in flight, then add(c) folds them in list order (the reference's order).
+template <class Load, class Add>
+__device__ __forceinline__ void walk_quads(uint32_t i, int m, const uint4* __restrict__ quads, Load&& load,
+                                           Add&& add) {
+    if (m <= 0) return;
+    const int nfull = m >> 2, rest = m & 3;
+    uint4 cur = quads[sweep_quad(i, 0)];
+    for (int q = 0; q < nfull; ++q) {
+        const auto c0 = load(cur.x);
+        const auto c1 = load(cur.y);
+        const auto c2 = load(cur.z);
+        const auto c3 = load(cur.w);
+        // quad q + 1 <= m/4 <= 62 exists in every particle's 63 (unused
+        // entries are never gathered)
+        cur = quads[sweep_quad(i, q + 1)];
+        add(c0);
+        add(c1);
+        add(c2);
+        add(c3);
+    }
+    if (rest) {
+        auto c0 = load(cur.x);
+        auto c1 = c0, c2 = c0;
+        if (rest > 1) c1 = load(cur.y);
+        if (rest > 2) c2 = load(cur.z);
+        add(c0);
+        if (rest > 1) add(c1);
+        if (rest > 2) add(c2);
+    }
+}
+
+template <int L_NUM, int L_DEN, class Load, class Add>
+__device__ __forceinline__ void walk_friends(uint32_t i, uint32_t packed, const uint4* __restrict__ quads, Load&& load,
+                                             Add&& add) {
+    int m;
+    (void)circles_processed<L_NUM, L_DEN>(packed, &m);
+    walk_quads(i, m, quads, load, add);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the chip; this

@@ -196,7 +196,7 @@ struct FriendWriter {
             ++dropped;
             return;
         }
-        nb_list[friend_seg(i, circ) + f] = j;
+        nb_list[friend_slot(i, circ, (int)f)] = j;
         packed += 1u << (6 * circ);
     }
     __device__ __forceinline__ void finish(uint32_t i, uint32_t* __restrict__ nb_cnt, uint32_t* err) {
@@ -220,20 +220,13 @@ __device__ __forceinline__ void pack_friends(uint32_t i, uint32_t packed, const 
     int na = 0, qo = 0;
     for (int q = 0; q < L; ++q) {
         const int cq = circle_count(packed, q);
-        const uint32_t* seg = nb_list + friend_seg(i, q);
-        for (int f = 0; f < cq; f += 4) {
-            const uint4 jj = *reinterpret_cast<const uint4*>(seg + f);
-            const uint32_t v4[4] = {jj.x, jj.y, jj.z, jj.w};
+        for (int f = 0; f < cq; ++f) {
+            const uint32_t j = nb_list[friend_slot(i, q, f)];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                if (f + t < cq) {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) acc[u] = (u == na) ? v4[t] : acc[u];
-                    if (++na == 4) {
-                        quads[sweep_quad(i, qo++)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
-                        na = 0;
-                    }
-                }
+            for (int u = 0; u < 4; ++u) acc[u] = (u == na) ? j : acc[u];
+            if (++na == 4) {
+                quads[sweep_quad(i, qo++)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
+                na = 0;
             }
         }
     }
@@ -414,23 +407,33 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
 // updates run in order.
 // ---------------------------------------------------------------------------
 
+// Two float4 of one friend (position and velocity or vorticity).
+struct Pair4 {
+    float4 a, b;
+};
+
 // compute_scaling.cl:1-113 + pack_data.cl (lambda into x*.w, in place: no
 // thread of this kernel reads another particle's w).
 struct LambdaAcc {
     float dens = 0.0f, gsum = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    // Branch-free: an out-of-range pair adds +0 (no sum here is ever -0, so
+    // the result is bitwise that of skipping it) and no branch joins the
+    // loop, which keeps the next gathers in flight.
     __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < c.h2) {
-            const float rl = sqrtf(r2);
-            const float hr = c.h - rl;
-            const float s = c.spiky * hr * hr;
-            const float ax = s * rx / rl, ay = s * ry / rl, az = s * rz / rl;
-            const float d = c.h2 - r2;
-            dens += d * d * d;
-            gsum += ax * ax + ay * ay + az * az;
-            gx += ax; gy += ay; gz += az;
-        }
+        const bool in = r2 < c.h2;
+        const float rl = sqrtf(r2);
+        const float hr = c.h - rl;
+        const float s = c.spiky * hr * hr;
+        const float inv = 1.0f / rl;   // r / |r| as r * (1/|r|)
+        const float ax = s * rx * inv, ay = s * ry * inv, az = s * rz * inv;
+        const float d = c.h2 - r2;
+        dens += in ? d * d * d : 0.0f;
+        gsum += in ? ax * ax + ay * ay + az * az : 0.0f;
+        gx += in ? ax : 0.0f;
+        gy += in ? ay : 0.0f;
+        gz += in ? az : 0.0f;
     }
 };
 
@@ -447,13 +450,8 @@ __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ s
     if (i >= be.z) return;
     const float4 p = xs[i];
     LambdaAcc a;
-    for_friends<3, 5>(
-        i, nb_cnt[i], nb_list,
-        [&](const uint4& jj) {
-            const float4 q0 = xs[jj.x], q1 = xs[jj.y], q2 = xs[jj.z], q3 = xs[jj.w];
-            a.add(p, q0, c); a.add(p, q1, c); a.add(p, q2, c); a.add(p, q3, c);
-        },
-        [&](uint32_t j) { a.add(p, xs[j], c); });
+    walk_friends<3, 5>(
+        i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; }, [&](const float4& q) { a.add(p, q, c); });
     const float dens = a.dens * c.poly6;
     rho[i] = dens;
     const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
@@ -472,17 +470,19 @@ struct DeltaAcc {
     __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < c.h2) {
-            const float rl = sqrtf(r2);
-            const float hr = c.h - rl;
-            const float ax = rx / rl * hr * hr, ay = ry / rl * hr * hr, az = rz / rl * hr * hr;
-            const float d = c.h2 - r2;
-            const float p6 = d * d * d;
-            const float ratio = p6 / c.poly6_q;
-            const float scorr = c.st_k * ratio * ratio * ratio * ratio;
-            const float coef = p.w + pj.w + scorr;
-            sx += coef * ax; sy += coef * ay; sz += coef * az;
-        }
+        const bool in = r2 < c.h2;
+        const float rl = sqrtf(r2);
+        const float hr = c.h - rl;
+        const float inv = 1.0f / rl;
+        const float ax = rx * inv * hr * hr, ay = ry * inv * hr * hr, az = rz * inv * hr * hr;
+        const float d = c.h2 - r2;
+        const float p6 = d * d * d;
+        const float ratio = p6 / c.poly6_q;
+        const float scorr = c.st_k * ratio * ratio * ratio * ratio;
+        const float coef = p.w + pj.w + scorr;
+        sx += in ? coef * ax : 0.0f;
+        sy += in ? coef * ay : 0.0f;
+        sz += in ? coef * az : 0.0f;
     }
 };
 
@@ -500,13 +500,8 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
     if (i >= be.z) return;
     const float4 p = xs[i];
     DeltaAcc a;
-    for_friends<1, 2>(
-        i, nb_cnt[i], nb_list,
-        [&](const uint4& jj) {
-            const float4 q0 = xs[jj.x], q1 = xs[jj.y], q2 = xs[jj.z], q3 = xs[jj.w];
-            a.add(p, q0, c); a.add(p, q1, c); a.add(p, q2, c); a.add(p, q3, c);
-        },
-        [&](uint32_t j) { a.add(p, xs[j], c); });
+    walk_friends<1, 2>(
+        i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; }, [&](const float4& q) { a.add(p, q, c); });
     float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
     float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
     float fz = p.z + (-c.spiky * a.sz) / c.rest_density;
@@ -542,18 +537,20 @@ struct ViscAcc {
                                         const SceneConsts& c) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < c.h2 && fabsf(pj.w) > 1e-8f) {
-            const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-            const float d = c.h2 - r2;
-            const float rl = sqrtf(r2);
-            const float hr = c.h - rl;
-            const float ax = rx / rl * hr * hr, ay = ry / rl * hr * hr, az = rz / rl * hr * hr;
-            ox += uy * az - uz * ay;
-            oy += uz * ax - ux * az;
-            oz += ux * ay - uy * ax;
-            const float p6 = d * d * d;
-            vsx += ux * p6; vsy += uy * p6; vsz += uz * p6;
-        }
+        const bool in = r2 < c.h2 && fabsf(pj.w) > 1e-8f;
+        const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+        const float d = c.h2 - r2;
+        const float rl = sqrtf(r2);
+        const float hr = c.h - rl;
+        const float inv = 1.0f / rl;
+        const float ax = rx * inv * hr * hr, ay = ry * inv * hr * hr, az = rz * inv * hr * hr;
+        ox += in ? uy * az - uz * ay : 0.0f;
+        oy += in ? uz * ax - ux * az : 0.0f;
+        oz += in ? ux * ay - uy * ax : 0.0f;
+        const float p6 = d * d * d;
+        vsx += in ? ux * p6 : 0.0f;
+        vsy += in ? uy * p6 : 0.0f;
+        vsz += in ? uz * p6 : 0.0f;
     }
 };
 
@@ -570,14 +567,9 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
     const float4 p = xs[i];
     const float4 vi = v_in[i];
     ViscAcc a;
-    for_friends<1, 2>(
-        i, nb_cnt[i], nb_list,
-        [&](const uint4& jj) {
-            const float4 q0 = xs[jj.x], q1 = xs[jj.y], q2 = xs[jj.z], q3 = xs[jj.w];
-            const float4 u0 = v_in[jj.x], u1 = v_in[jj.y], u2 = v_in[jj.z], u3 = v_in[jj.w];
-            a.add(p, vi, q0, u0, c); a.add(p, vi, q1, u1, c); a.add(p, vi, q2, u2, c); a.add(p, vi, q3, u3, c);
-        },
-        [&](uint32_t j) { a.add(p, vi, xs[j], v_in[j], c); });
+    walk_friends<1, 2>(
+        i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], v_in[j]}; },
+        [&](const Pair4& t) { a.add(p, vi, t.a, t.b, c); });
     float4 vo;
     vo.x = vi.x + c.viscosity * (a.vsx * c.poly6);
     vo.y = vi.y + c.viscosity * (a.vsy * c.poly6);
@@ -593,13 +585,15 @@ struct VortAcc {
     __device__ __forceinline__ void add(const float4& p, const float4& pj, const float4& oj, const SceneConsts& c) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < c.h2 && fabsf(pj.w) > 1e-8f) {
-            const float rl = sqrtf(r2);
-            const float hr = c.h - rl;
-            const float ax = rx / rl * hr * hr, ay = ry / rl * hr * hr, az = rz / rl * hr * hr;
-            const float wl = sqrtf(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
-            ex += wl * ax; ey += wl * ay; ez += wl * az;
-        }
+        const bool in = r2 < c.h2 && fabsf(pj.w) > 1e-8f;
+        const float rl = sqrtf(r2);
+        const float hr = c.h - rl;
+        const float inv = 1.0f / rl;
+        const float ax = rx * inv * hr * hr, ay = ry * inv * hr * hr, az = rz * inv * hr * hr;
+        const float wl = sqrtf(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
+        ex += in ? wl * ax : 0.0f;
+        ey += in ? wl * ay : 0.0f;
+        ez += in ? wl * az : 0.0f;
     }
 };
 
@@ -615,14 +609,9 @@ __global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict_
     if (i >= be.z) return;
     const float4 p = xs[i];
     VortAcc a;
-    for_friends<1, 2>(
-        i, nb_cnt[i], nb_list,
-        [&](const uint4& jj) {
-            const float4 q0 = xs[jj.x], q1 = xs[jj.y], q2 = xs[jj.z], q3 = xs[jj.w];
-            const float4 o0 = omega[jj.x], o1 = omega[jj.y], o2 = omega[jj.z], o3 = omega[jj.w];
-            a.add(p, q0, o0, c); a.add(p, q1, o1, c); a.add(p, q2, o2, c); a.add(p, q3, o3, c);
-        },
-        [&](uint32_t j) { a.add(p, xs[j], omega[j], c); });
+    walk_friends<1, 2>(
+        i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], omega[j]}; },
+        [&](const Pair4& t) { a.add(p, t.a, t.b, c); });
     float ex = a.ex * -c.spiky, ey = a.ey * -c.spiky, ez = a.ez * -c.spiky;
     const float l = sqrtf(ex * ex + ey * ey + ez * ez);
     if (l > 0.0f) { ex /= l; ey /= l; ez /= l; }

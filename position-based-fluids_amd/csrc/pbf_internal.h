@@ -117,7 +117,7 @@ struct pbf_ctx {
     // grid and friends
     uint32_t* cells = nullptr;       // uint2[total_cells]
     uint32_t* nb_cnt = nullptr;      // packed 5x6-bit circle counts
-    uint32_t* nb_list = nullptr;     // friends, [n][5 circles][kSeg]
+    uint32_t* nb_list = nullptr;     // friends, [n/64][5 circles][kSeg][64]
     uint4* nb_quads = nullptr;       // processed friends, sweep quads
 
     uint32_t* err = nullptr;

@@ -102,14 +102,19 @@ __device__ __forceinline__ int circles_processed(uint32_t packed, int* m) {
 // Friends lists: particle i owns, for each of its five circles, kSeg slots
 // holding its circle-c friends in the reference's scan order -- the
 // reference's layout (counts[5][n], indices[5][50][n],
-// build_friends_list.cl:1-21) blocked per 64 consecutive particles: slot f of
-// circle c of the block's 64 particles is one contiguous 256-byte row
-// [(i>>6)][c][f][lane], so the appends of a wave's lanes land in the same few
-// rows (a small write footprint that stays in L2) and a wave's reads of one
-// slot are coalesced.
-constexpr int kSeg = kPerCircle;
-__device__ __host__ __forceinline__ size_t friend_slot(uint32_t i, int circle, int f) {
-    return (((size_t)(i >> 6) * kCircles + (size_t)circle) * kSeg + (size_t)f) * 64 + (i & 63u);
+// build_friends_list.cl:1-21) blocked per 64 consecutive particles and
+// grouped in quads: quad q (slots 4q..4q+3) of circle c of the block's 64
+// particles is one contiguous 1 KB row [(i>>6)][c][q][lane] of uint4.  The
+// build stores a quad when it fills (16-byte stores that land, for all the
+// lanes of a wave, in the same few rows) and the regrouping reads it back
+// with one load.
+constexpr int kSegQuads = (kPerCircle + 3) / 4;   // 13
+constexpr int kSeg = 4 * kSegQuads;               // 52 slots per circle
+__device__ __host__ __forceinline__ size_t friend_quad(uint32_t i, int circle, int q) {
+    return (((size_t)(i >> 6) * kCircles + (size_t)circle) * kSegQuads + (size_t)q) * 64 + (i & 63u);
+}
+__device__ __host__ __forceinline__ size_t friend_slot(uint32_t i, int circle, int f) {   // in uint32 units
+    return friend_quad(i, circle, f >> 2) * 4 + (size_t)(f & 3);
 }
 __device__ __host__ __forceinline__ size_t friend_slots(size_t n) { return ((n + 63) / 64) * 64 * kCircles * kSeg; }
 

@@ -187,19 +187,36 @@ __device__ __forceinline__ int friend_circle(const float4& p, const float4& q, c
 }
 
 // Appends friend j of circle `circ` to particle i's list under the 50 cap.
+// Each lane stages the quad it is filling for every circle in its own LDS
+// column (entry (c, s) at stage[(4c + s) * BS]) and stores the quad when it
+// fills; finish() stores the partial ones (entries past a count are never
+// read).
+template <int BS>
 struct FriendWriter {
     uint32_t packed = 0, dropped = 0;
-    __device__ __forceinline__ void take(uint32_t i, uint32_t j, int circ, uint32_t* __restrict__ nb_list) {
+    uint32_t* stage = nullptr;
+    __device__ __forceinline__ void take(uint32_t i, uint32_t j, int circ, uint4* __restrict__ list) {
         if (circ < 0 || j == i) return;
         const uint32_t f = (packed >> (6 * circ)) & 63u;
         if (f >= (uint32_t)kPerCircle) {
             ++dropped;
             return;
         }
-        nb_list[friend_slot(i, circ, (int)f)] = j;
+        uint32_t* col = stage + 4 * circ * BS;
+        col[(f & 3u) * BS] = j;
+        if ((f & 3u) == 3u) list[friend_quad(i, circ, (int)(f >> 2))] = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
         packed += 1u << (6 * circ);
     }
-    __device__ __forceinline__ void finish(uint32_t i, uint32_t* __restrict__ nb_cnt, uint32_t* err) {
+    __device__ __forceinline__ void finish(uint32_t i, uint32_t* __restrict__ nb_cnt, uint4* __restrict__ list,
+                                           uint32_t* err) {
+#pragma unroll
+        for (int q = 0; q < kCircles; ++q) {
+            const int f = circle_count(packed, q);
+            if (f & 3) {
+                const uint32_t* col = stage + 4 * q * BS;
+                list[friend_quad(i, q, f >> 2)] = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
+            }
+        }
         nb_cnt[i] = packed;
         if (dropped) {
             atomicOr(err, (uint32_t)PBF_FLAG_FRIENDS_OVERFLOW);
@@ -209,10 +226,10 @@ struct FriendWriter {
 };
 
 // Regroups particle i's processed friends (circles of the 0.6 prefix, which
-// contains the 0.5 prefix) from its circle segments into the sweep quads.
-// Run by the thread that has just written the segments, so they are read
-// back from cache.
-__device__ __forceinline__ void pack_friends(uint32_t i, uint32_t packed, const uint32_t* __restrict__ nb_list,
+// contains the 0.5 prefix) from its circle quads into the sweep quads.  Run
+// by the thread that has just written the circle quads, so they are read back
+// from cache.
+__device__ __forceinline__ void pack_friends(uint32_t i, uint32_t packed, const uint4* __restrict__ list,
                                              uint4* __restrict__ quads) {
     int m;
     const int L = circles_processed<3, 5>(packed, &m);
@@ -220,13 +237,19 @@ __device__ __forceinline__ void pack_friends(uint32_t i, uint32_t packed, const 
     int na = 0, qo = 0;
     for (int q = 0; q < L; ++q) {
         const int cq = circle_count(packed, q);
-        for (int f = 0; f < cq; ++f) {
-            const uint32_t j = nb_list[friend_slot(i, q, f)];
+        for (int f = 0; f < cq; f += 4) {
+            const uint4 jj = list[friend_quad(i, q, f >> 2)];
+            const uint32_t v4[4] = {jj.x, jj.y, jj.z, jj.w};
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] = (u == na) ? j : acc[u];
-            if (++na == 4) {
-                quads[sweep_quad(i, qo++)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
-                na = 0;
+            for (int t = 0; t < 4; ++t) {
+                if (f + t < cq) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) acc[u] = (u == na) ? v4[t] : acc[u];
+                    if (++na == 4) {
+                        quads[sweep_quad(i, qo++)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
+                        na = 0;
+                    }
+                }
             }
         }
     }
@@ -234,9 +257,10 @@ __device__ __forceinline__ void pack_friends(uint32_t i, uint32_t packed, const 
 }
 
 // Scans the global bucket [b.x, b.y] for particle i, four candidates at a time.
+template <int BS>
 __device__ __forceinline__ void scan_bucket_global(const float4& p, uint32_t i, uint2 b, uint32_t n,
                                                    const float4* __restrict__ xs, const SceneConsts& c,
-                                                   uint32_t* __restrict__ nb_list, FriendWriter& w) {
+                                                   uint4* __restrict__ nb_list, FriendWriter<BS>& w) {
     if (b.x == kEmpty) return;
     const uint32_t e = min(b.y, n - 1);
     uint32_t j = b.x;
@@ -256,19 +280,22 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
                                                       const uint2* __restrict__ cells, uint32_t* __restrict__ nb_cnt,
                                                       uint32_t* __restrict__ nb_list, uint4* __restrict__ quads,
                                                       uint32_t n, uint32_t* err) {
+    __shared__ uint32_t s_stage[4 * kCircles * BS];
     const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
     const SceneConsts& c = scenes[be.x];
     const uint32_t i = be.y + threadIdx.x;
     if (i >= be.z) return;
+    uint4* list = reinterpret_cast<uint4*>(nb_list);
     const float4 p = xs[i];
     const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
-    FriendWriter w;
+    FriendWriter<BS> w;
+    w.stage = s_stage + threadIdx.x;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz)
-                scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, nb_list, w);
-    w.finish(i, nb_cnt, err);
-    pack_friends(i, w.packed, nb_list, quads);
+                scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list, w);
+    w.finish(i, nb_cnt, list, err);
+    pack_friends(i, w.packed, list, quads);
 }
 
 // Staging budget of the LDS variant per workgroup size: box cells, particles.
@@ -292,6 +319,7 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
     __shared__ uint32_t s_off[kMaxCells + 1];
     __shared__ int s_red[6][kWaves];
     __shared__ uint32_t s_scan[kWaves];
+    __shared__ uint32_t s_stage[4 * kCircles * BS];
 
     const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
     const SceneConsts& c = scenes[be.x];
@@ -375,7 +403,9 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
     if (!staged && tid == 0) atomicAdd(err + 4, 1u);
     if (!valid) return;
 
-    FriendWriter w;
+    uint4* list = reinterpret_cast<uint4*>(nb_list);
+    FriendWriter<BS> w;
+    w.stage = s_stage + tid;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
@@ -388,16 +418,16 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
                         const float4 q2 = s_pos[off + k + 2], q3 = s_pos[off + k + 3];
                         const int c0 = friend_circle(p, q0, c), c1 = friend_circle(p, q1, c);
                         const int c2 = friend_circle(p, q2, c), c3 = friend_circle(p, q3, c);
-                        w.take(i, gs + k, c0, nb_list); w.take(i, gs + k + 1, c1, nb_list);
-                        w.take(i, gs + k + 2, c2, nb_list); w.take(i, gs + k + 3, c3, nb_list);
+                        w.take(i, gs + k, c0, list); w.take(i, gs + k + 1, c1, list);
+                        w.take(i, gs + k + 2, c2, list); w.take(i, gs + k + 3, c3, list);
                     }
-                    for (; k < cnt; ++k) w.take(i, gs + k, friend_circle(p, s_pos[off + k], c), nb_list);
+                    for (; k < cnt; ++k) w.take(i, gs + k, friend_circle(p, s_pos[off + k], c), list);
                 } else {
-                    scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, nb_list, w);
+                    scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list, w);
                 }
             }
-    w.finish(i, nb_cnt, err);
-    pack_friends(i, w.packed, nb_list, quads);
+    w.finish(i, nb_cnt, list, err);
+    pack_friends(i, w.packed, list, quads);
 }
 
 // ---------------------------------------------------------------------------

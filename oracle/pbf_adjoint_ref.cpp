@@ -156,7 +156,7 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
     const float poly6_qf = tq * tq * tq;
     const T h = K<T>::c(hf), h2 = K<T>::c(h2f), dt = K<T>::c(c.dt), rd = K<T>::c(c.rest_density);
     const T eps = K<T>::c(c.epsilon), stk = K<T>::c(c.st_k), poly6 = K<T>::c(poly6f), spiky = K<T>::c(spikyf);
-    const T poly6_q = K<T>::c(poly6_qf);
+    const T inv_q = K<T>::c(1.0f / poly6_qf);
     const T visc = coef[0], vort = coef[1];
     const T g[3] = {coef[2], coef[3], coef[4]};
     const T zero = K<T>::c(0.0f), one = K<T>::c(1.0f), five = K<T>::c(5.0f), three = K<T>::c(3.0f);
@@ -230,7 +230,7 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                         const T gg[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
                         const T d = h2 - r2;
                         const T p6 = d * d * d;
-                        const T ratio = p6 / poly6_q;
+                        const T ratio = p6 * inv_q;
                         const T scorr = stk * ratio * ratio * ratio * ratio;
                         const T co = lam[i] + lam[j] + scorr;
                         sum[0] += co * gg[0]; sum[1] += co * gg[1]; sum[2] += co * gg[2];

@@ -332,6 +332,10 @@ void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const u
     const float q2 = dqh * dqh;
     const float tq = h2 - q2;
     const float poly6_q = tq * tq * tq;
+    /* poly6_r / poly6_q (compute_delta.cl:151) as poly6_r * (1/poly6_q): the
+     * divisor is a constant of the step, so one correctly rounded reciprocal
+     * serves every pair (relaxed math permits the same in the reference) */
+    const float inv_q = 1.0f / poly6_q;
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
         const uint32_t i = (uint32_t)ii;
@@ -354,7 +358,7 @@ void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const u
                     const float g[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
                     const float d = h2 - r2;
                     const float p6 = d * d * d;
-                    const float ratio = p6 / poly6_q;
+                    const float ratio = p6 * inv_q;
                     const float scorr = K * ratio * ratio * ratio * ratio;
                     const float coef = pi[3] + pj[3] + scorr;
                     sum[0] += coef * g[0]; sum[1] += coef * g[1]; sum[2] += coef * g[2];

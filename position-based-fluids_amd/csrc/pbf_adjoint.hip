@@ -61,29 +61,48 @@ __device__ __forceinline__ V3 cross(V3 a, V3 b) {
 }
 __device__ __forceinline__ float4 f4(V3 a) { return make_float4(a.x, a.y, a.z, 0.0f); }
 
-// Geometry of an in-range pair.
+// Geometry of a pair, evaluated whether or not it is in range (the sweeps
+// below are branch-free: out-of-range pairs contribute a selected zero).
 struct Pair {
     V3 r;
-    float r2, q, hr, d;
+    float r2, q, inv, hr, d;
+    bool in;
 };
 
-__device__ __forceinline__ bool pair_of(const float4& a, const float4& b, const SceneConsts& c, Pair& g) {
+__device__ __forceinline__ Pair pair_of(const float4& a, const float4& b, const SceneConsts& c) {
+    Pair g;
     g.r = {a.x - b.x, a.y - b.y, a.z - b.z};
     g.r2 = dot(g.r, g.r);
-    if (!(g.r2 < c.h2)) return false;
+    g.in = g.r2 < c.h2;
     g.q = sqrtf(g.r2);
+    g.inv = 1.0f / g.q;
     g.hr = c.h - g.q;
     g.d = c.h2 - g.r2;
-    return true;
+    return g;
 }
 
-__device__ __forceinline__ V3 psi(const Pair& g) { return g.r * (g.hr * g.hr / g.q); }
+__device__ __forceinline__ V3 psi(const Pair& g) { return g.r * (g.hr * g.hr * g.inv); }
 
 __device__ __forceinline__ V3 jpsi(const Pair& g, V3 u, const SceneConsts& c) {
-    const float w = g.hr * g.hr / g.q;
-    const float wq = -(g.hr * (c.h + g.q)) / (g.r2 * g.q);   // w'/q
+    const float w = g.hr * g.hr * g.inv;
+    const float wq = -(g.hr * (c.h + g.q)) * (g.inv * g.inv * g.inv);   // w'/q
     return u * w + g.r * (wq * dot(g.r, u));
 }
+
+__device__ __forceinline__ V3 sel(bool m, V3 v) {
+    return {m ? v.x : 0.0f, m ? v.y : 0.0f, m ? v.z : 0.0f};
+}
+
+struct G2 {
+    float4 a, b;
+};
+struct G4 {
+    float4 a, b, c, d;
+};
+struct G2f {
+    float4 a, b;
+    float f;
+};
 
 // Transposed entries: count of the 0.6 list in the low half, of its 0.5
 // prefix in the high half.
@@ -292,16 +311,13 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b1(const SceneConsts* __re
     if (i < be.z) {
         const float4 p = P[i];
         V3 eta = {0.f, 0.f, 0.f};
-        auto add = [&](uint32_t j) {
-            const float4 pj = P[j];
-            Pair g;
-            if (pair_of(p, pj, c, g) && fabsf(pj.w) > 1e-8f) {
-                const V3 oj = v3(om[j]);
-                eta += psi(g) * sqrtf(dot(oj, oj));
-            }
-        };
-        for_friends<1, 2>(
-            i, cnt[i], quads, [&](const uint4& jj) { add(jj.x); add(jj.y); add(jj.z); add(jj.w); }, add);
+        walk_friends<1, 2>(
+            i, cnt[i], quads, [&](uint32_t j) { return G2{P[j], om[j]}; },
+            [&](const G2& t) {
+                const Pair g = pair_of(p, t.a, c);
+                const V3 oj = v3(t.b);
+                eta += sel(g.in && fabsf(t.a.w) > 1e-8f, psi(g) * sqrtf(dot(oj, oj)));
+            });
         eta = eta * -c.spiky;
         const float l = sqrtf(dot(eta, eta));
         const V3 N = l > 0.0f ? eta * (1.0f / l) : eta;
@@ -332,29 +348,25 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b2(
     const float4 p = P[m];
     const V3 zm = v3(z[m]);
     V3 gP = {0.f, 0.f, 0.f};
-    auto own = [&](uint32_t j) {
-        const float4 pj = P[j];
-        Pair g;
-        if (pair_of(p, pj, c, g) && fabsf(pj.w) > 1e-8f) {
-            const V3 oj = v3(om[j]);
-            gP += jpsi(g, zm, c) * sqrtf(dot(oj, oj));
-        }
-    };
-    for_friends<1, 2>(
-        m, cnt[m], quads, [&](const uint4& jj) { own(jj.x); own(jj.y); own(jj.z); own(jj.w); }, own);
+    walk_friends<1, 2>(
+        m, cnt[m], quads, [&](uint32_t j) { return G2{P[j], om[j]}; },
+        [&](const G2& t) {
+            const Pair g = pair_of(p, t.a, c);
+            const V3 oj = v3(t.b);
+            gP += sel(g.in && fabsf(t.a.w) > 1e-8f, jpsi(g, zm, c) * sqrtf(dot(oj, oj)));
+        });
     if (fabsf(p.w) > 1e-8f) {
         const V3 omm = v3(om[m]);
         const float wl = sqrtf(dot(omm, omm));
         float gs = 0.0f;
-        auto tr = [&](uint32_t i) {
-            Pair g;
-            if (pair_of(P[i], p, c, g)) {
-                const V3 zi = v3(z[i]);
-                gP -= jpsi(g, zi, c) * wl;
-                gs += dot(psi(g), zi);
-            }
-        };
-        for_quads(m, t05(tcnt[m]), tquads, [&](const uint4& ii) { tr(ii.x); tr(ii.y); tr(ii.z); tr(ii.w); }, tr);
+        walk_quads(
+            m, t05(tcnt[m]), tquads, [&](uint32_t i) { return G2{P[i], z[i]}; },
+            [&](const G2& t) {
+                const Pair g = pair_of(t.a, p, c);
+                const V3 zi = v3(t.b);
+                gP -= sel(g.in, jpsi(g, zi, c) * wl);
+                gs += g.in ? dot(psi(g), zi) : 0.0f;
+            });
         if (wl > 0.0f) gom[m] = f4(v3(gom[m]) + omm * (gs / wl));
     }
     gx[m] = f4(v3(gx[m]) * scale[be.x] + gP);
@@ -383,35 +395,30 @@ __global__ __launch_bounds__(kAdjThreads) void k_xsph_b(
         const V3 am = gvm * cp6;
         const V3 gam = v3(gom[m]) * -c.spiky;
         V3 S = {0.f, 0.f, 0.f}, gVu = {0.f, 0.f, 0.f}, gP = {0.f, 0.f, 0.f};
-        auto own = [&](uint32_t j) {
-            const float4 pj = P[j];
-            Pair g;
-            if (pair_of(p, pj, c, g) && fabsf(pj.w) > 1e-8f) {
-                const V3 u = v3(vu[j]) - vm;
+        walk_friends<1, 2>(
+            m, cnt[m], quads, [&](uint32_t j) { return G2{P[j], vu[j]}; },
+            [&](const G2& t) {
+                const Pair g = pair_of(p, t.a, c);
+                const bool in = g.in && fabsf(t.a.w) > 1e-8f;
+                const V3 u = v3(t.b) - vm;
                 const float W = g.d * g.d * g.d;
-                const V3 ps = psi(g);
-                S += u * W;
-                gVu -= am * W + cross(ps, gam);
-                gP += g.r * (-6.0f * g.d * g.d * dot(u, am)) + jpsi(g, cross(gam, u), c);
-            }
-        };
-        for_friends<1, 2>(
-            m, cnt[m], quads, [&](const uint4& jj) { own(jj.x); own(jj.y); own(jj.z); own(jj.w); }, own);
+                S += sel(in, u * W);
+                gVu -= sel(in, am * W + cross(psi(g), gam));
+                gP += sel(in, g.r * (-6.0f * g.d * g.d * dot(u, am)) + jpsi(g, cross(gam, u), c));
+            });
         gc[0] = dot(S * c.poly6, gvm);
         if (fabsf(p.w) > 1e-8f) {
-            auto tr = [&](uint32_t i) {
-                Pair g;
-                if (pair_of(P[i], p, c, g)) {
-                    const V3 u = vm - v3(vu[i]);
-                    const V3 ai = (v3(gv[i]) * sc) * cp6;
-                    const V3 gai = v3(gom[i]) * -c.spiky;
+            walk_quads(
+                m, t05(tcnt[m]), tquads, [&](uint32_t i) { return G4{P[i], vu[i], gv[i], gom[i]}; },
+                [&](const G4& t) {
+                    const Pair g = pair_of(t.a, p, c);
+                    const V3 u = vm - v3(t.b);
+                    const V3 ai = (v3(t.c) * sc) * cp6;
+                    const V3 gai = v3(t.d) * -c.spiky;
                     const float W = g.d * g.d * g.d;
-                    gVu += ai * W + cross(psi(g), gai);
-                    gP -= g.r * (-6.0f * g.d * g.d * dot(u, ai)) + jpsi(g, cross(gai, u), c);
-                }
-            };
-            for_quads(m, t05(tcnt[m]), tquads, [&](const uint4& ii) { tr(ii.x); tr(ii.y); tr(ii.z); tr(ii.w); },
-                      tr);
+                    gVu += sel(g.in, ai * W + cross(psi(g), gai));
+                    gP -= sel(g.in, g.r * (-6.0f * g.d * g.d * dot(u, ai)) + jpsi(g, cross(gai, u), c));
+                });
         }
         gVu += gvm;
         const V3 gPK = v3(gx[m]) + gP + gVu * (1.0f / c.dt);
@@ -438,39 +445,34 @@ __global__ __launch_bounds__(kAdjThreads) void k_delta_b(
     if (m >= be.z) return;
     const float4 p = P[m];
     const V3 km = v3(z[m]);
-    const float ks = -24.0f * c.st_k / c.poly6_q;
+    const float ks = -24.0f * c.st_k * c.inv_poly6_q;
     float gl = 0.0f;
     V3 gP = {0.f, 0.f, 0.f};
-    auto own = [&](uint32_t j) {
-        const float4 pj = P[j];
-        Pair g;
-        if (pair_of(p, pj, c, g)) {
+    walk_friends<1, 2>(
+        m, cnt[m], quads, [&](uint32_t j) { return P[j]; },
+        [&](const float4& pj) {
+            const Pair g = pair_of(p, pj, c);
             const V3 ps = psi(g);
-            const float ratio = g.d * g.d * g.d / c.poly6_q;
+            const float ratio = g.d * g.d * g.d * c.inv_poly6_q;
             const float r3 = ratio * ratio * ratio;
             const float coef = p.w + pj.w + c.st_k * r3 * ratio;
             const float t = dot(ps, km);
-            gl += t;
-            gP += jpsi(g, km, c) * coef + g.r * (t * ks * r3 * g.d * g.d);
-        }
-    };
-    for_friends<1, 2>(
-        m, cnt[m], quads, [&](const uint4& jj) { own(jj.x); own(jj.y); own(jj.z); own(jj.w); }, own);
-    auto tr = [&](uint32_t i) {
-        const float4 pi = P[i];
-        Pair g;
-        if (pair_of(pi, p, c, g)) {
-            const V3 ki = v3(z[i]);
+            gl += g.in ? t : 0.0f;
+            gP += sel(g.in, jpsi(g, km, c) * coef + g.r * (t * ks * r3 * g.d * g.d));
+        });
+    walk_quads(
+        m, t05(tcnt[m]), tquads, [&](uint32_t i) { return G2{P[i], z[i]}; },
+        [&](const G2& t) {
+            const Pair g = pair_of(t.a, p, c);
+            const V3 ki = v3(t.b);
             const V3 ps = psi(g);
-            const float ratio = g.d * g.d * g.d / c.poly6_q;
+            const float ratio = g.d * g.d * g.d * c.inv_poly6_q;
             const float r3 = ratio * ratio * ratio;
-            const float coef = pi.w + p.w + c.st_k * r3 * ratio;
-            const float t = dot(ps, ki);
-            gl += t;
-            gP -= jpsi(g, ki, c) * coef + g.r * (t * ks * r3 * g.d * g.d);
-        }
-    };
-    for_quads(m, t05(tcnt[m]), tquads, [&](const uint4& ii) { tr(ii.x); tr(ii.y); tr(ii.z); tr(ii.w); }, tr);
+            const float coef = t.a.w + p.w + c.st_k * r3 * ratio;
+            const float tt = dot(ps, ki);
+            gl += g.in ? tt : 0.0f;
+            gP -= sel(g.in, jpsi(g, ki, c) * coef + g.r * (tt * ks * r3 * g.d * g.d));
+        });
     gx[m] = f4(v3(gx[m]) + gP);
     const float4 a4 = aux[m];
     const float den = a4.w;
@@ -507,25 +509,20 @@ __global__ __launch_bounds__(kAdjThreads) void k_lambda_b(
         const V3 Hm = v3(qm);
         const float Am = qm.w, Bm = zb[m];
         V3 gP = {0.f, 0.f, 0.f};
-        auto own = [&](uint32_t j) {
-            Pair g;
-            if (pair_of(p, P[j], c, g)) {
+        walk_friends<3, 5>(
+            m, cnt[m], quads, [&](uint32_t j) { return P[j]; },
+            [&](const float4& pj) {
+                const Pair g = pair_of(p, pj, c);
                 const V3 ps = psi(g);
-                gP += g.r * (6.0f * Am * g.d * g.d) + jpsi(g, ps * (Bm * c.spiky) + Hm, c);
-            }
-        };
-        for_friends<3, 5>(
-            m, cnt[m], quads, [&](const uint4& jj) { own(jj.x); own(jj.y); own(jj.z); own(jj.w); }, own);
-        auto tr = [&](uint32_t i) {
-            Pair g;
-            if (pair_of(P[i], p, c, g)) {
-                const float4 qi = z2[i];
-                const float Bi = zb[i];
+                gP += sel(g.in, g.r * (6.0f * Am * g.d * g.d) + jpsi(g, ps * (Bm * c.spiky) + Hm, c));
+            });
+        walk_quads(
+            m, t06(tcnt[m]), tquads, [&](uint32_t i) { return G2f{P[i], z2[i], zb[i]}; },
+            [&](const G2f& t) {
+                const Pair g = pair_of(t.a, p, c);
                 const V3 ps = psi(g);
-                gP -= g.r * (6.0f * qi.w * g.d * g.d) + jpsi(g, ps * (Bi * c.spiky) + v3(qi), c);
-            }
-        };
-        for_quads(m, t06(tcnt[m]), tquads, [&](const uint4& ii) { tr(ii.x); tr(ii.y); tr(ii.z); tr(ii.w); }, tr);
+                gP -= sel(g.in, g.r * (6.0f * t.b.w * g.d * g.d) + jpsi(g, ps * (t.f * c.spiky) + v3(t.b), c));
+            });
         const V3 gPk = v3(gx[m]) + gP;
         if (!FIRST) {
             const V3 gF = wall_vjp(fut_prev[m], gPk, c);

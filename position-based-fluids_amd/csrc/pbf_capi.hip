@@ -153,7 +153,7 @@ pbf::SceneConsts make_consts(const pbf_params& P, const pbf_options& o, uint32_t
     const float dqh = P.surfaceTenstionDist * P.h;   // compute_delta.cl:81-82
     const float q2 = dqh * dqh;
     const float tq = P.h_2 - q2;
-    c.poly6_q = tq * tq * tq;
+    c.inv_poly6_q = 1.0f / (tq * tq * tq);   // the ratio's divisor as one reciprocal per step
     c.viscosity = P.viscosityFactor;
     c.vorticity = P.vorticityFactor;
     c.lo0 = P.xMin; c.lo1 = P.yMin; c.lo2 = P.zMin;

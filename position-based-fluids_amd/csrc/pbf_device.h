@@ -20,7 +20,7 @@ struct SceneConsts {
     float h, h2, min_r, span;       // span = h - 0.3h (build_friends_list.cl:32,75)
     float dt, g0, g1, g2;
     float rest_density, epsilon, poly6, spiky;  // poly6/spiky: the 6-digit factors
-    float st_k, poly6_q, viscosity, vorticity;  // s_corr constants (compute_delta.cl:81-82,152)
+    float st_k, inv_poly6_q, viscosity, vorticity;  // s_corr constants (compute_delta.cl:81-82,151-152)
     float lo0, lo1, lo2, hi0;
     float hi1, hi2;
     uint32_t walls, gbs;

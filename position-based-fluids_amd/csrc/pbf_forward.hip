@@ -507,7 +507,7 @@ struct DeltaAcc {
         const float ax = rx * inv * hr * hr, ay = ry * inv * hr * hr, az = rz * inv * hr * hr;
         const float d = c.h2 - r2;
         const float p6 = d * d * d;
-        const float ratio = p6 / c.poly6_q;
+        const float ratio = p6 * c.inv_poly6_q;
         const float scorr = c.st_k * ratio * ratio * ratio * ratio;
         const float coef = p.w + pj.w + scorr;
         sx += in ? coef * ax : 0.0f;

@@ -199,8 +199,9 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                         const T rl = t_sqrt(r2);
                         const T hr = h - rl;
                         const T sp = spiky * hr * hr;
-                        const T inv = one / rl;
-                        const T gg[3] = {sp * r[0] * inv, sp * r[1] * inv, sp * r[2] * inv};
+                        const T inv = val(r2) > 0.0 ? one / rl : zero;
+                        const T t = sp * inv;
+                        const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                         const T d = h2 - r2;
                         dens += d * d * d;
                         gsum += gg[0] * gg[0] + gg[1] * gg[1] + gg[2] * gg[2];
@@ -226,8 +227,9 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                     if (val(r2) < val(h2)) {
                         const T rl = t_sqrt(r2);
                         const T hr = h - rl;
-                        const T inv = one / rl;
-                        const T gg[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
+                        const T inv = val(r2) > 0.0 ? one / rl : zero;
+                        const T t = inv * hr * hr;
+                        const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                         const T d = h2 - r2;
                         const T p6 = d * d * d;
                         const T ratio = p6 * inv_q;
@@ -273,8 +275,9 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                     const T d = h2 - r2;
                     const T rl = t_sqrt(r2);
                     const T hr = h - rl;
-                    const T inv = one / rl;
-                        const T gg[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
+                    const T inv = val(r2) > 0.0 ? one / rl : zero;
+                        const T t = inv * hr * hr;
+                        const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                     o[0] += u[1] * gg[2] - u[2] * gg[1];
                     o[1] += u[2] * gg[0] - u[0] * gg[2];
                     o[2] += u[0] * gg[1] - u[1] * gg[0];
@@ -301,8 +304,9 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                 if (val(r2) < val(h2) && fabs(val(rho[j])) > (double)1e-8f) {
                     const T rl = t_sqrt(r2);
                     const T hr = h - rl;
-                    const T inv = one / rl;
-                        const T gg[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
+                    const T inv = val(r2) > 0.0 ? one / rl : zero;
+                        const T t = inv * hr * hr;
+                        const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const T* oj = om[j].a;
                     const T wl = t_sqrt(oj[0] * oj[0] + oj[1] * oj[1] + oj[2] * oj[2]);
                     eta[0] += wl * gg[0]; eta[1] += wl * gg[1]; eta[2] += wl * gg[2];

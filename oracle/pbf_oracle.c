@@ -256,8 +256,9 @@ void pbfo_scaling(const pbfo_cfg *c, const float *xs, const uint32_t *fr, float 
                     const float rl = sqrtf(r2);
                     const float hr = h - rl;
                     const float s = spiky * hr * hr;
-                    const float inv = 1.0f / rl;   /* r / |r| as r * (1/|r|) */
-                    const float g[3] = {s * r[0] * inv, s * r[1] * inv, s * r[2] * inv};
+                    const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 at r = 0 */
+                    const float t = s * inv;
+                    const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const float d = h2 - r2;
                     dens += d * d * d;
                     gsum += dot3(g, g);
@@ -354,8 +355,9 @@ void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const u
                 if (r2 < h2) {
                     const float rl = sqrtf(r2);
                     const float hr = h - rl;
-                    const float inv = 1.0f / rl;   /* r / |r| as r * (1/|r|) */
-                    const float g[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
+                    const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 at r = 0 */
+                    const float t = inv * hr * hr;
+                    const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const float d = h2 - r2;
                     const float p6 = d * d * d;
                     const float ratio = p6 * inv_q;
@@ -450,8 +452,9 @@ void pbfo_viscosity(const pbfo_cfg *c, const float *xs, const float *v_in, const
                     const float d = h2 - r2;
                     const float rl = sqrtf(r2);
                     const float hr = h - rl;
-                    const float inv = 1.0f / rl;   /* r / |r| as r * (1/|r|) */
-                    const float g[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
+                    const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 at r = 0 */
+                    const float t = inv * hr * hr;
+                    const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     float cr[3];
                     cross3(vij, g, cr);
                     om[0] += cr[0]; om[1] += cr[1]; om[2] += cr[2];
@@ -494,8 +497,9 @@ void pbfo_vorticity(const pbfo_cfg *c, const float *xs, float *v, const float *o
                 if (r2 < h2 && fabsf(pj[3]) > 1e-8f) {
                     const float rl = sqrtf(r2);
                     const float hr = h - rl;
-                    const float inv = 1.0f / rl;   /* r / |r| as r * (1/|r|) */
-                    const float g[3] = {r[0] * inv * hr * hr, r[1] * inv * hr * hr, r[2] * inv * hr * hr};
+                    const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 at r = 0 */
+                    const float t = inv * hr * hr;
+                    const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const float *oj = F4(omega, j);
                     const float wl = sqrtf(dot3(oj, oj));
                     eta[0] += wl * g[0]; eta[1] += wl * g[1]; eta[2] += wl * g[2];

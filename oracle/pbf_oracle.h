@@ -15,7 +15,10 @@
  * applies to every friend sweep: the vector r / |r| (compute_scaling.cl:83,
  * compute_delta.cl:144, apply_viscosity.cl:65, apply_vorticity.cl) is
  * evaluated as r * (1/|r|), one correctly rounded reciprocal per pair, which
- * the relaxed-math build of the reference also permits.
+ * the relaxed-math build of the reference also permits; at r = 0 (coincident
+ * particles) the reference's 0/0 gives NaN, which then spreads through every
+ * later step, and here the direction is 0 (the pair still counts in the
+ * density; its gradient terms vanish).
  *
  * Data layout (all little-endian, particle-major):
  *   float4 arrays   : float[4*n]   (x, y, z, w)

@@ -75,7 +75,7 @@ __device__ __forceinline__ Pair pair_of(const float4& a, const float4& b, const 
     g.r2 = dot(g.r, g.r);
     g.in = g.r2 < c.h2;
     g.q = sqrtf(g.r2);
-    g.inv = 1.0f / g.q;
+    g.inv = g.r2 > 0.0f ? 1.0f / g.q : 0.0f;   // direction of a coincident pair: 0
     g.hr = c.h - g.q;
     g.d = c.h2 - g.r2;
     return g;
@@ -175,13 +175,15 @@ __device__ __forceinline__ V3 wall_vjp(const float4& fut, V3 g, const SceneConst
 // Recording
 // ---------------------------------------------------------------------------
 
-// T_m from m's full friend list (all circles, particle-major segments): the
+// T_m from m's full friend list (all circles: circle 0 from the front of
+// the sweep quads, the others from the circle quads): the
 // friends i whose processed prefix contains m's circle, 0.5 members first.
 __global__ __launch_bounds__(kAdjThreads) void k_transpose(const SceneConsts* __restrict__ scenes,
                                                           const uint4* __restrict__ blocks,
                                                           const float4* __restrict__ xs,
                                                           const uint32_t* __restrict__ cnt,
                                                           const uint32_t* __restrict__ nb_list,
+                                                          const uint32_t* __restrict__ sweep,
                                                           const uint2* __restrict__ cells,
                                                           uint32_t* __restrict__ tcnt, uint4* __restrict__ tquads,
                                                           uint32_t* flags) {
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_transpose(const SceneConsts* __
         for (int q = 0; q < kCircles; ++q) {
             const int cq = circle_count(pm, q);
             for (int f = 0; f < cq; ++f) {
-                const uint32_t i = nb_list[friend_slot(m, q, f)];
+                const uint32_t i = friend_at(nb_list, sweep, m, q, f);
                 const uint32_t pi = cnt[i];
                 int dummy;
                 const int l05 = circles_processed<1, 2>(pi, &dummy);
@@ -703,6 +705,7 @@ int tape_step(pbf_adjoint* a, const TapeSlot& t, unsigned long long* note) {
         ctx->omega = om0;
     };
     ctx->pause_now = 0;
+    ctx->cells_fresh = true;
     hipError_t e = launch_clear(ctx);
     if (e == hipSuccess) e = launch_predict_hash(ctx, ctx->cells_hold_keys);
     ctx->perm = t.perm;
@@ -718,8 +721,8 @@ int tape_step(pbf_adjoint* a, const TapeSlot& t, unsigned long long* note) {
     if (e == hipSuccess) {
         const BlockTable& bt = ctx->bt[2];
         hipLaunchKernelGGL(k_transpose, dim3(bt.count), dim3(kAdjThreads), 0, ctx->stream, ctx->sc_d, bt.d, t.P,
-                           t.cnt, ctx->nb_list, reinterpret_cast<const uint2*>(ctx->cells), t.tcnt, t.tquads,
-                           a->flags);
+                           t.cnt, ctx->nb_list, reinterpret_cast<const uint32_t*>(t.quads),
+                           reinterpret_cast<const uint2*>(ctx->cells), t.tcnt, t.tquads, a->flags);
         hipLaunchKernelGGL(k_note_drops, dim3(1), dim3(64), 0, ctx->stream, ctx->err, note);
         e = hipGetLastError();
     }

@@ -310,6 +310,7 @@ int enqueue_step(pbf_ctx* ctx, uint32_t pause) {
     // Simulation.cpp:563-567: predict always; radixsort unless paused.  With
     // the particle order not sorted (no step yet, or new state) the step
     // sorts anyway.
+    ctx->cells_fresh = !(pause && ctx->order_sorted);
     if (pause && ctx->order_sorted) {
         HIP_TRY(ctx, pbf::launch_predict_only(ctx, 1u));
     } else {
@@ -561,6 +562,8 @@ int pbf_get_structure(pbf_ctx* ctx, uint32_t* keys_sorted, uint32_t* perm, uint3
         HIP_TRY(ctx, hipMemcpy(packed.data(), ctx->nb_cnt, 4 * n, hipMemcpyDeviceToHost));
         std::vector<uint32_t> slots(pbf::friend_slots(n));
         HIP_TRY(ctx, hipMemcpy(slots.data(), ctx->nb_list, 4 * slots.size(), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> sweep(((n + 63) / 64) * 64 * (size_t)pbf::kQuads * 4);
+        HIP_TRY(ctx, hipMemcpy(sweep.data(), ctx->nb_quads, 4 * sweep.size(), hipMemcpyDeviceToHost));
         uint32_t* idx = friends + (size_t)pbf::kCircles * n;
         memset(idx, 0, 4 * n * pbf::kCircles * pbf::kPerCircle);
         for (size_t i = 0; i < n; ++i)
@@ -568,7 +571,8 @@ int pbf_get_structure(pbf_ctx* ctx, uint32_t* keys_sorted, uint32_t* perm, uint3
                 const uint32_t cnt = (packed[i] >> (6 * q)) & 63u;
                 friends[(size_t)q * n + i] = cnt;
                 for (uint32_t f = 0; f < cnt; ++f)
-                    idx[((size_t)q * pbf::kPerCircle + f) * n + i] = slots[pbf::friend_slot((uint32_t)i, q, (int)f)];
+                    idx[((size_t)q * pbf::kPerCircle + f) * n + i] =
+                        pbf::friend_at(slots.data(), sweep.data(), (uint32_t)i, q, (int)f);
             }
     }
     return PBF_OK;

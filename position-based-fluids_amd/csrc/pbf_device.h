@@ -124,8 +124,17 @@ __device__ __host__ __forceinline__ size_t friend_slots(size_t n) { return ((n +
 // particles as one uint4 per particle, so a wave's load of one quad is a
 // contiguous 1 KB and gives every lane four friends.
 constexpr int kQuads = (kCircles * kPerCircle + 3) / 4;
-__device__ __forceinline__ size_t sweep_quad(uint32_t i, int q) {
+__device__ __host__ __forceinline__ size_t sweep_quad(uint32_t i, int q) {
     return ((size_t)(i >> 6) * kQuads + (size_t)q) * 64 + (i & 63u);
+}
+
+// Circle 0 is processed by every sweep (0 processed < any threshold), so the
+// build writes it straight to the front of the sweep quads; circles 1..4 go
+// to the circle quads and only the processed ones are appended after it.
+// Friend f of circle c of particle i, from the two (as uint32 arrays):
+__device__ __host__ __forceinline__ uint32_t friend_at(const uint32_t* list, const uint32_t* sweep, uint32_t i,
+                                                       int c, int f) {
+    return c == 0 ? sweep[sweep_quad(i, f >> 2) * 4 + (size_t)(f & 3)] : list[friend_slot(i, c, f)];
 }
 
 // Walks the first m entries of particle i's list in quad layout, four per

@@ -195,7 +195,12 @@ template <int BS>
 struct FriendWriter {
     uint32_t packed = 0, dropped = 0;
     uint32_t* stage = nullptr;
-    __device__ __forceinline__ void take(uint32_t i, uint32_t j, int circ, uint4* __restrict__ list) {
+    __device__ __forceinline__ static uint4* quad_of(uint32_t i, int circ, int q, uint4* __restrict__ list,
+                                                      uint4* __restrict__ sweep) {
+        return circ == 0 ? sweep + sweep_quad(i, q) : list + friend_quad(i, circ, q);
+    }
+    __device__ __forceinline__ void take(uint32_t i, uint32_t j, int circ, uint4* __restrict__ list,
+                                         uint4* __restrict__ sweep) {
         if (circ < 0 || j == i) return;
         const uint32_t f = (packed >> (6 * circ)) & 63u;
         if (f >= (uint32_t)kPerCircle) {
@@ -204,17 +209,17 @@ struct FriendWriter {
         }
         uint32_t* col = stage + 4 * circ * BS;
         col[(f & 3u) * BS] = j;
-        if ((f & 3u) == 3u) list[friend_quad(i, circ, (int)(f >> 2))] = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
+        if ((f & 3u) == 3u) *quad_of(i, circ, (int)(f >> 2), list, sweep) = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
         packed += 1u << (6 * circ);
     }
     __device__ __forceinline__ void finish(uint32_t i, uint32_t* __restrict__ nb_cnt, uint4* __restrict__ list,
-                                           uint32_t* err) {
+                                           uint4* __restrict__ sweep, uint32_t* err) {
 #pragma unroll
         for (int q = 0; q < kCircles; ++q) {
             const int f = circle_count(packed, q);
             if (f & 3) {
                 const uint32_t* col = stage + 4 * q * BS;
-                list[friend_quad(i, q, f >> 2)] = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
+                *quad_of(i, q, f >> 2, list, sweep) = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
             }
         }
         nb_cnt[i] = packed;
@@ -225,17 +230,23 @@ struct FriendWriter {
     }
 };
 
-// Regroups particle i's processed friends (circles of the 0.6 prefix, which
-// contains the 0.5 prefix) from its circle quads into the sweep quads.  Run
-// by the thread that has just written the circle quads, so they are read back
-// from cache.
+// Appends particle i's processed circles after circle 0 (already in place)
+// in the sweep quads.  Run by the thread that has just written the circle
+// quads, so they are read back from cache; with circle 0 alone processed
+// there is nothing to do.
 __device__ __forceinline__ void pack_friends(uint32_t i, uint32_t packed, const uint4* __restrict__ list,
                                              uint4* __restrict__ quads) {
     int m;
     const int L = circles_processed<3, 5>(packed, &m);
+    if (L <= 1) return;
+    const int n0 = circle_count(packed, 0);
     uint32_t acc[4] = {0u, 0u, 0u, 0u};
-    int na = 0, qo = 0;
-    for (int q = 0; q < L; ++q) {
+    int na = n0 & 3, qo = n0 >> 2;
+    if (na) {
+        const uint4 t = quads[sweep_quad(i, qo)];
+        acc[0] = t.x; acc[1] = t.y; acc[2] = t.z; acc[3] = t.w;
+    }
+    for (int q = 1; q < L; ++q) {
         const int cq = circle_count(packed, q);
         for (int f = 0; f < cq; f += 4) {
             const uint4 jj = list[friend_quad(i, q, f >> 2)];
@@ -260,7 +271,8 @@ __device__ __forceinline__ void pack_friends(uint32_t i, uint32_t packed, const 
 template <int BS>
 __device__ __forceinline__ void scan_bucket_global(const float4& p, uint32_t i, uint2 b, uint32_t n,
                                                    const float4* __restrict__ xs, const SceneConsts& c,
-                                                   uint4* __restrict__ nb_list, FriendWriter<BS>& w) {
+                                                   uint4* __restrict__ nb_list, uint4* __restrict__ sweep,
+                                                   FriendWriter<BS>& w) {
     if (b.x == kEmpty) return;
     const uint32_t e = min(b.y, n - 1);
     uint32_t j = b.x;
@@ -268,10 +280,54 @@ __device__ __forceinline__ void scan_bucket_global(const float4& p, uint32_t i, 
         const float4 q0 = xs[j], q1 = xs[j + 1], q2 = xs[j + 2], q3 = xs[j + 3];
         const int c0 = friend_circle(p, q0, c), c1 = friend_circle(p, q1, c);
         const int c2 = friend_circle(p, q2, c), c3 = friend_circle(p, q3, c);
-        w.take(i, j, c0, nb_list); w.take(i, j + 1, c1, nb_list);
-        w.take(i, j + 2, c2, nb_list); w.take(i, j + 3, c3, nb_list);
+        w.take(i, j, c0, nb_list, sweep); w.take(i, j + 1, c1, nb_list, sweep);
+        w.take(i, j + 2, c2, nb_list, sweep); w.take(i, j + 3, c3, nb_list, sweep);
     }
-    for (; j <= e; ++j) w.take(i, j, friend_circle(p, xs[j], c), nb_list);
+    for (; j <= e; ++j) w.take(i, j, friend_circle(p, xs[j], c), nb_list, sweep);
+}
+
+// Neighbour cells that cannot hold a friend.  Cell k covers [k h, (k+1) h)
+// for k > 0, (-h, h) for k = 0 and ((k-1) h, k h] for k < 0 (truncating
+// conversion); a neighbour whose box is farther than h from the particle
+// holds no particle within h.  Skipping it leaves the list unchanged provided
+// its hash bucket is not also reached through another of the 27 offsets (the
+// reference then lists that bucket's friends twice): with gridBufSize = 2^b,
+// b >= 6, the 27 keys always differ (the cell coordinates differ mod 4), and
+// particles of far cells folded into a bucket are never within h.  Returns
+// bit (dx+1)*9 + (dy+1)*3 + (dz+1) set for each cell to scan; the 1e-4
+// margin covers the rounding of x/h, k*h and the gap sums.
+__device__ __forceinline__ float cell_lo(int k, float h) {
+    return k > 0 ? (float)k * h : (k == 0 ? -h : (float)(k - 1) * h);
+}
+__device__ __forceinline__ float cell_hi(int k, float h) { return k >= 0 ? (float)(k + 1) * h : (float)k * h; }
+
+__device__ __forceinline__ uint32_t cells_to_scan(const float4& p, int32_t cx, int32_t cy, int32_t cz,
+                                                  const SceneConsts& c, uint32_t fresh) {
+    // a paused step keeps the previous step's buckets (no sort): particles
+    // need not lie in the box of the cell they are listed under
+    if (!fresh || !(c.gbs_mask != 0u && c.gbs >= 64u)) return (1u << 27) - 1u;
+    const float h = c.h;
+    float g[3][3];
+    const float pc[3] = {p.x, p.y, p.z};
+    const int32_t cc[3] = {cx, cy, cz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float lo = fmaxf(pc[a] - cell_hi(cc[a] - 1, h), 0.0f);
+        const float hi = fmaxf(cell_lo(cc[a] + 1, h) - pc[a], 0.0f);
+        g[a][0] = lo * lo;
+        g[a][1] = 0.0f;
+        g[a][2] = hi * hi;
+    }
+    const float lim = c.h2 * 1.0001f;
+    uint32_t mask = 0u;
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+#pragma unroll
+            for (int z = 0; z < 3; ++z)
+                mask |= (uint32_t)(!(g[0][x] + g[1][y] + g[2][z] > lim)) << (x * 9 + y * 3 + z);
+    return mask;
 }
 
 template <int BS>
@@ -279,7 +335,7 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
                                                       const uint4* __restrict__ blocks, const float4* __restrict__ xs,
                                                       const uint2* __restrict__ cells, uint32_t* __restrict__ nb_cnt,
                                                       uint32_t* __restrict__ nb_list, uint4* __restrict__ quads,
-                                                      uint32_t n, uint32_t* err) {
+                                                      uint32_t n, uint32_t fresh, uint32_t* err) {
     __shared__ uint32_t s_stage[4 * kCircles * BS];
     const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
     const SceneConsts& c = scenes[be.x];
@@ -290,11 +346,14 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
     const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
     FriendWriter<BS> w;
     w.stage = s_stage + threadIdx.x;
+    const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz)
-                scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list, w);
-    w.finish(i, nb_cnt, list, err);
+                if (scan & (1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1))))
+                    scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list, quads,
+                                       w);
+    w.finish(i, nb_cnt, list, quads, err);
     pack_friends(i, w.packed, list, quads);
 }
 
@@ -309,7 +368,7 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
                                                const float4* __restrict__ xs, const uint2* __restrict__ cells,
                                                uint32_t* __restrict__ nb_cnt, uint32_t* __restrict__ nb_list,
-                                               uint4* __restrict__ quads, uint32_t n, uint32_t* err) {
+                                               uint4* __restrict__ quads, uint32_t n, uint32_t fresh, uint32_t* err) {
     constexpr int kWaves = BS / 64;
     constexpr int kMaxCells = FriendsBudget<BS>::cells;
     constexpr int kMaxStaged = FriendsBudget<BS>::staged;
@@ -406,9 +465,11 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
     uint4* list = reinterpret_cast<uint4*>(nb_list);
     FriendWriter<BS> w;
     w.stage = s_stage + tid;
+    const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
+                if (!(scan & (1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1))))) continue;
                 if (staged) {
                     const int q = (cx + dx - bx) + ddx * ((cy + dy - by) + ddy * (cz + dz - bz));
                     const uint32_t off = s_off[q], cnt = s_off[q + 1] - off, gs = s_start[q];
@@ -418,15 +479,16 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
                         const float4 q2 = s_pos[off + k + 2], q3 = s_pos[off + k + 3];
                         const int c0 = friend_circle(p, q0, c), c1 = friend_circle(p, q1, c);
                         const int c2 = friend_circle(p, q2, c), c3 = friend_circle(p, q3, c);
-                        w.take(i, gs + k, c0, list); w.take(i, gs + k + 1, c1, list);
-                        w.take(i, gs + k + 2, c2, list); w.take(i, gs + k + 3, c3, list);
+                        w.take(i, gs + k, c0, list, quads); w.take(i, gs + k + 1, c1, list, quads);
+                        w.take(i, gs + k + 2, c2, list, quads); w.take(i, gs + k + 3, c3, list, quads);
                     }
-                    for (; k < cnt; ++k) w.take(i, gs + k, friend_circle(p, s_pos[off + k], c), list);
+                    for (; k < cnt; ++k) w.take(i, gs + k, friend_circle(p, s_pos[off + k], c), list, quads);
                 } else {
-                    scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list, w);
+                    scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list, quads,
+                                       w);
                 }
             }
-    w.finish(i, nb_cnt, list, err);
+    w.finish(i, nb_cnt, list, quads, err);
     pack_friends(i, w.packed, list, quads);
 }
 
@@ -446,9 +508,10 @@ struct Pair4 {
 // thread of this kernel reads another particle's w).
 struct LambdaAcc {
     float dens = 0.0f, gsum = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    // Branch-free: an out-of-range pair adds +0 (no sum here is ever -0, so
-    // the result is bitwise that of skipping it) and no branch joins the
-    // loop, which keeps the next gathers in flight.
+    // Branch-free: an out-of-range pair has its scalar factors set to 0, so it
+    // adds +0 or -0 (no sum here is ever -0, so the result is bitwise that of
+    // skipping it) and no branch joins the loop, which keeps the next gathers
+    // in flight.
     __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -456,14 +519,15 @@ struct LambdaAcc {
         const float rl = sqrtf(r2);
         const float hr = c.h - rl;
         const float s = c.spiky * hr * hr;
-        const float inv = 1.0f / rl;   // r / |r| as r * (1/|r|)
-        const float ax = s * rx * inv, ay = s * ry * inv, az = s * rz * inv;
-        const float d = c.h2 - r2;
-        dens += in ? d * d * d : 0.0f;
-        gsum += in ? ax * ax + ay * ay + az * az : 0.0f;
-        gx += in ? ax : 0.0f;
-        gy += in ? ay : 0.0f;
-        gz += in ? az : 0.0f;
+        const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   // r/|r| = r * (1/|r|); 0 at r = 0
+        const float t = in ? s * inv : 0.0f;
+        const float ax = rx * t, ay = ry * t, az = rz * t;
+        const float d = in ? c.h2 - r2 : 0.0f;
+        dens += d * d * d;
+        gsum += ax * ax + ay * ay + az * az;
+        gx += ax;
+        gy += ay;
+        gz += az;
     }
 };
 
@@ -503,16 +567,17 @@ struct DeltaAcc {
         const bool in = r2 < c.h2;
         const float rl = sqrtf(r2);
         const float hr = c.h - rl;
-        const float inv = 1.0f / rl;
-        const float ax = rx * inv * hr * hr, ay = ry * inv * hr * hr, az = rz * inv * hr * hr;
+        const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;
+        const float t = inv * hr * hr;
+        const float ax = rx * t, ay = ry * t, az = rz * t;
         const float d = c.h2 - r2;
         const float p6 = d * d * d;
         const float ratio = p6 * c.inv_poly6_q;
         const float scorr = c.st_k * ratio * ratio * ratio * ratio;
-        const float coef = p.w + pj.w + scorr;
-        sx += in ? coef * ax : 0.0f;
-        sy += in ? coef * ay : 0.0f;
-        sz += in ? coef * az : 0.0f;
+        const float coef = in ? p.w + pj.w + scorr : 0.0f;
+        sx += coef * ax;
+        sy += coef * ay;
+        sz += coef * az;
     }
 };
 
@@ -567,20 +632,23 @@ struct ViscAcc {
                                         const SceneConsts& c) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
+        // masked through the geometric factors, which are finite whenever
+        // the pair is skipped (a skipped pair at r = 0 would give 0 * inf)
         const bool in = r2 < c.h2 && fabsf(pj.w) > 1e-8f;
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float d = c.h2 - r2;
         const float rl = sqrtf(r2);
         const float hr = c.h - rl;
-        const float inv = 1.0f / rl;
-        const float ax = rx * inv * hr * hr, ay = ry * inv * hr * hr, az = rz * inv * hr * hr;
-        ox += in ? uy * az - uz * ay : 0.0f;
-        oy += in ? uz * ax - ux * az : 0.0f;
-        oz += in ? ux * ay - uy * ax : 0.0f;
-        const float p6 = d * d * d;
-        vsx += in ? ux * p6 : 0.0f;
-        vsy += in ? uy * p6 : 0.0f;
-        vsz += in ? uz * p6 : 0.0f;
+        const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;
+        const float t = in ? inv * hr * hr : 0.0f;
+        const float ax = rx * t, ay = ry * t, az = rz * t;
+        ox += uy * az - uz * ay;
+        oy += uz * ax - ux * az;
+        oz += ux * ay - uy * ax;
+        const float p6 = in ? d * d * d : 0.0f;
+        vsx += ux * p6;
+        vsy += uy * p6;
+        vsz += uz * p6;
     }
 };
 
@@ -618,12 +686,13 @@ struct VortAcc {
         const bool in = r2 < c.h2 && fabsf(pj.w) > 1e-8f;
         const float rl = sqrtf(r2);
         const float hr = c.h - rl;
-        const float inv = 1.0f / rl;
-        const float ax = rx * inv * hr * hr, ay = ry * inv * hr * hr, az = rz * inv * hr * hr;
+        const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;
+        const float t = in ? inv * hr * hr : 0.0f;
+        const float ax = rx * t, ay = ry * t, az = rz * t;
         const float wl = sqrtf(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
-        ex += in ? wl * ax : 0.0f;
-        ey += in ? wl * ay : 0.0f;
-        ez += in ? wl * az : 0.0f;
+        ex += wl * ax;
+        ey += wl * ay;
+        ez += wl * az;
     }
 };
 
@@ -710,17 +779,18 @@ hipError_t launch_cells(pbf_ctx* ctx) {
 
 hipError_t launch_friends(pbf_ctx* ctx) {
     track_begin(ctx, "buildFriendsList");
+    const uint32_t fresh = ctx->cells_fresh ? 1u : 0u;
     const BlockTable& t = table_for(ctx, F_FRIENDS);
     if (ctx->friends_staged) {
         PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS],
                         hipLaunchKernelGGL(k_friends<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d,
                                            ctx->xs[ctx->xs_cur], reinterpret_cast<const uint2*>(ctx->cells),
-                                           ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, ctx->err));
+                                           ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, fresh, ctx->err));
     } else {
         PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS],
                         hipLaunchKernelGGL(k_friends_direct<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
                                            t.d, ctx->xs[ctx->xs_cur], reinterpret_cast<const uint2*>(ctx->cells),
-                                           ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, ctx->err));
+                                           ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, fresh, ctx->err));
     }
     track_end(ctx);
     return hipGetLastError();

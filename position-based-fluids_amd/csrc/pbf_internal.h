@@ -126,6 +126,7 @@ struct pbf_ctx {
     bool have_state = false;
     bool cells_hold_keys = false;  // cells hold the entries of keys_sorted (reset them first)
     bool order_sorted = false;     // particle order == keys_sorted order (pause may skip the sort)
+    bool cells_fresh = false;      // this step's cells were built from its own predicted positions
     float4* x_base = nullptr;      // buffer identities for the graph keys
     float4* v_base = nullptr;
     bool profiling = false;

@@ -80,6 +80,34 @@ int pbf_scene_offsets(pbf_ctx *ctx, uint32_t *offsets /* scenes + 1 */);
  * (zero velocities). */
 int pbf_set_state(pbf_ctx *ctx, const float *x4, const float *v4, uint32_t n);
 
+/* One collision quad: corner B, edges E0 and E1 (BouncePointQuad's
+ * arguments, compute_delta.cl:1) and the band of mask rows it samples:
+ * texel row mask_y + (int)(s * mask_h) at quad parameter s along E0. */
+typedef struct pbf_quad {
+    float origin[3];
+    float e0[3];
+    float e1[3];
+    int32_t mask_y, mask_h;
+} pbf_quad;
+
+#define PBF_MAX_QUADS 26
+
+/* The reference's force plates (compute_delta.cl:1-42, applied after the
+ * walls as at :169-182, with the surface mask that Simulation::LoadForceMasks
+ * loads, src/Simulation.cpp:257-281): in the delta stage each quad in turn
+ * moves a point that is inside its parallelogram, behind its plane offset by
+ * edge_base + edge and within the force zone, onto that offset plane, where
+ * the mask texel ((int)(t mask_w), mask_y + (int)(s mask_h)), clamped to the
+ * image, is nonzero.  edge is the walls' noise edge; edge_base is the
+ * reference's "1.0 +" (compute_delta.cl:166).  mask: mask_w x mask_rows
+ * texels, row-major (the reference's CL_R / CL_UNSIGNED_INT32 image of the
+ * PNG's red channel).  count = 0 removes the scene's quads (quads and mask may
+ * then be NULL).  count <= PBF_MAX_QUADS; every quad needs finite entries
+ * and non-parallel edges.  Replaces the scene's previous quads; waits for the
+ * context's queued work. */
+int pbf_set_collision_quads(pbf_ctx *ctx, uint32_t scene, const pbf_quad *quads, uint32_t count,
+                            const uint32_t *mask, uint32_t mask_w, uint32_t mask_rows, float edge_base);
+
 /* Simulation::Step (src/Simulation.cpp:554-621).  pause mirrors bPauseSim:
  * no gravity and no re-sort. */
 int pbf_step(pbf_ctx *ctx, uint32_t pause);

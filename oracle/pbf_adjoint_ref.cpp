@@ -82,7 +82,7 @@ struct Frozen {
     uint32_t n, iters, steps;
     const uint32_t* perm;      // steps x n
     const uint32_t* friends;   // steps x (5 + 250) n, reference layout
-    const uint8_t* hits;       // steps x iters x n
+    const uint32_t* hits;      // steps x iters x n: planes that moved each particle
 };
 
 inline uint32_t fr_count(const uint32_t* fr, uint32_t n, int q, uint32_t i) { return fr[(size_t)q * n + i]; }
@@ -162,6 +162,26 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
     const T zero = K<T>::c(0.0f), one = K<T>::c(1.0f), five = K<T>::c(5.0f), three = K<T>::c(3.0f);
     const T c003 = K<T>::c(0.03f), minus1 = K<T>::c(-1.0f);
 
+    // collision quads with the constants BouncePointQuad derives (binary32,
+    // as the oracle computes them)
+    struct QuadF {
+        float b[3], e0[3], e1[3], n[3], aa, bb, cc, invdet;
+    };
+    const uint32_t qn = c.quads && c.mask ? (c.quad_count < 26 ? c.quad_count : 26) : 0;
+    std::vector<QuadF> qs(qn);
+    for (uint32_t k = 0; k < qn; ++k) {
+        const pbfo_quad& q = c.quads[k];
+        QuadF& o = qs[k];
+        for (int a = 0; a < 3; ++a) { o.b[a] = q.b[a]; o.e0[a] = q.e0[a]; o.e1[a] = q.e1[a]; }
+        o.aa = q.e0[0] * q.e0[0] + q.e0[1] * q.e0[1] + q.e0[2] * q.e0[2];
+        o.bb = q.e0[0] * q.e1[0] + q.e0[1] * q.e1[1] + q.e0[2] * q.e1[2];
+        o.cc = q.e1[0] * q.e1[0] + q.e1[1] * q.e1[1] + q.e1[2] * q.e1[2];
+        o.invdet = 1.0f / (o.aa * o.cc - o.bb * o.bb);
+        const float cr[3] = {q.e0[1] * q.e1[2] - q.e0[2] * q.e1[1], q.e0[2] * q.e1[0] - q.e0[0] * q.e1[2],
+                             q.e0[0] * q.e1[1] - q.e0[1] * q.e1[0]};
+        const float il = 1.0f / sqrtf(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        for (int a = 0; a < 3; ++a) o.n[a] = cr[a] * il;
+    }
     std::vector<V<T>>& x = st.x;
     std::vector<V<T>>& v = st.v;
     std::vector<V<T>> xs(n), xs2(n), tmp(n), om(n);
@@ -184,7 +204,7 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
         x.swap(tmp);
         st.ids.swap(ids2);
         for (uint32_t it = 0; it < f.iters; ++it) {
-            const uint8_t* hit = f.hits + ((size_t)s * f.iters + it) * n;
+            const uint32_t* hit = f.hits + ((size_t)s * f.iters + it) * n;
             // compute_scaling.cl
 #pragma omp parallel for schedule(static)
             for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
@@ -240,12 +260,25 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                 });
                 T fut[3];
                 for (int k = 0; k < 3; ++k) fut[k] = pi[k] + (minus1 * spiky * sum[k]) / rd;
-                if (c.walls && hit[i]) {
+                if (hit[i]) {
                     const T ns[3] = {t_sin(fut[0] * five / h), t_sin(fut[1] * five / h), t_sin(fut[2] * five / h)};
                     const T edge = (three + ns[0] + ns[1] + ns[2]) * h * c003;
                     for (int a = 0; a < 3; ++a) {
                         if (hit[i] & (1u << (2 * a))) fut[a] = K<T>::c(c.box_lo[a]) + edge;
                         if (hit[i] & (2u << (2 * a))) fut[a] = K<T>::c(c.box_hi[a]) - edge;
+                    }
+                    // quads (BouncePointQuad): the projection onto the plane, offset
+                    const T eq = K<T>::c(c.edge_base) + edge;
+                    for (uint32_t k = 0; k < qn; ++k) {
+                        if (!(hit[i] & (1u << (6 + k)))) continue;
+                        const QuadF& q = qs[k];
+                        const T D[3] = {K<T>::c(q.b[0]) - fut[0], K<T>::c(q.b[1]) - fut[1], K<T>::c(q.b[2]) - fut[2]};
+                        const T e = K<T>::c(q.e1[0]) * D[0] + K<T>::c(q.e1[1]) * D[1] + K<T>::c(q.e1[2]) * D[2];
+                        const T d = K<T>::c(q.e0[0]) * D[0] + K<T>::c(q.e0[1]) * D[1] + K<T>::c(q.e0[2]) * D[2];
+                        const T ss = K<T>::c(q.invdet) * (K<T>::c(q.bb) * e - K<T>::c(q.cc) * d);
+                        const T tt = K<T>::c(q.invdet) * (K<T>::c(q.bb) * d - K<T>::c(q.aa) * e);
+                        for (int a = 0; a < 3; ++a)
+                            fut[a] = K<T>::c(q.b[a]) + K<T>::c(q.e0[a]) * ss + K<T>::c(q.e1[a]) * tt + K<T>::c(q.n[a]) * eq;
                     }
                 }
                 for (int k = 0; k < 3; ++k) xs2[i].a[k] = pi[k] + (fut[k] - pi[k]);
@@ -414,7 +447,7 @@ void tangent_fd(const pbfo_cfg& c, const Frozen& f, const float* x0, const float
 extern "C" {
 
 void pbfa_tangent_fd(const pbfo_cfg* c, uint32_t steps, const uint32_t* perm, const uint32_t* friends,
-                     const uint8_t* hits, const float* x0, const float* v0, const float* coef, const double* dx0,
+                     const uint32_t* hits, const float* x0, const float* v0, const float* coef, const double* dx0,
                      const double* dv0, const double* dcoef, double delta, double* dxT, double* dvT) {
     const Frozen f{c->n, c->iters, steps, perm, friends, hits};
     tangent_fd(*c, f, x0, v0, coef, dx0, dv0, dcoef, delta, dxT, dvT);
@@ -424,21 +457,21 @@ void pbfa_tangent_fd(const pbfo_cfg* c, uint32_t steps, const uint32_t* perm, co
 // Structure arrays: perm[steps][n], friends[steps][(5 + 250) n] (the
 // reference layout of pbfo_buffer(.., 9)), hits[steps][iters][n].
 void pbfa_rollout_f64(const pbfo_cfg* c, uint32_t steps, const uint32_t* perm, const uint32_t* friends,
-                      const uint8_t* hits, const double* x0, const double* v0, const double* coef, double* xT,
+                      const uint32_t* hits, const double* x0, const double* v0, const double* coef, double* xT,
                       double* vT) {
     const Frozen f{c->n, c->iters, steps, perm, friends, hits};
     rollout<double>(*c, f, x0, v0, coef, xT, vT);
 }
 
 void pbfa_jvp_f64(const pbfo_cfg* c, uint32_t steps, const uint32_t* perm, const uint32_t* friends,
-                  const uint8_t* hits, const double* x0, const double* v0, const double* coef, const double* dx0,
+                  const uint32_t* hits, const double* x0, const double* v0, const double* coef, const double* dx0,
                   const double* dv0, const double* dcoef, double* xT, double* vT, double* dxT, double* dvT) {
     const Frozen f{c->n, c->iters, steps, perm, friends, hits};
     jvp<double>(*c, f, x0, v0, coef, dx0, dv0, dcoef, xT, vT, dxT, dvT);
 }
 
 void pbfa_jvp_f32(const pbfo_cfg* c, uint32_t steps, const uint32_t* perm, const uint32_t* friends,
-                  const uint8_t* hits, const float* x0, const float* v0, const float* coef, const double* dx0,
+                  const uint32_t* hits, const float* x0, const float* v0, const float* coef, const double* dx0,
                   const double* dv0, const double* dcoef, float* xT, float* vT, double* dxT, double* dvT) {
     const Frozen f{c->n, c->iters, steps, perm, friends, hits};
     jvp<float>(*c, f, x0, v0, coef, dx0, dv0, dcoef, xT, vT, dxT, dvT);

@@ -188,6 +188,14 @@ void pbfo_cells(const uint32_t *ks, uint32_t n, uint32_t *cells)
 
 static inline float dot3(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
+static inline void cross3(const float *a, const float *b, float *o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+
 /* build_friends_list.cl:23-103 */
 void pbfo_friends(const pbfo_cfg *c, const float *xs, const uint32_t *cells, uint32_t *fr)
 {
@@ -313,6 +321,62 @@ static inline int wall_hi(float *f, const float *prev, int a, float hi, float ed
     return 1;
 }
 
+/* A quad with the constants BouncePointQuad derives from it
+ * (compute_delta.cl:3-9), computed once per call. */
+typedef struct {
+    float b[3], e0[3], e1[3], n[3];
+    float aa, bb, cc, invdet;
+    int32_t mask_y, mask_h;
+} quadc;
+
+static void quad_consts(const pbfo_quad *q, quadc *o)
+{
+    for (int k = 0; k < 3; ++k) { o->b[k] = q->b[k]; o->e0[k] = q->e0[k]; o->e1[k] = q->e1[k]; }
+    o->aa = dot3(q->e0, q->e0);
+    o->bb = dot3(q->e0, q->e1);
+    o->cc = dot3(q->e1, q->e1);
+    /* the reference's 1.0 is a double literal; restated in binary32 */
+    o->invdet = 1.0f / (o->aa * o->cc - o->bb * o->bb);
+    float cr[3];
+    cross3(q->e0, q->e1, cr);
+    /* normalize() as v * (1/|v|) */
+    const float il = 1.0f / sqrtf(dot3(cr, cr));
+    for (int k = 0; k < 3; ++k) o->n[k] = cr[k] * il;
+    o->mask_y = q->mask_y;
+    o->mask_h = q->mask_h;
+}
+
+/* BouncePointQuad (compute_delta.cl:1-42): a point behind the quad's
+ * offset plane, inside the quad and within the force zone, where the mask is
+ * set, moves onto the offset plane.  Texel (int)(t W), mask_y + (int)(s h),
+ * clamped to the image (CLK_ADDRESS_CLAMP_TO_EDGE, utilities.cl:89). */
+static int bounce_quad(const quadc *q, const float *prev, float *f, float edge, const uint32_t *mask,
+                       uint32_t mw, uint32_t mrows)
+{
+    const float D[3] = {q->b[0] - f[0], q->b[1] - f[1], q->b[2] - f[2]};
+    const float e = dot3(q->e1, D);
+    const float d = dot3(q->e0, D);
+    const float s = q->invdet * (q->bb * e - q->cc * d);
+    const float t = q->invdet * (q->bb * d - q->aa * e);
+    if ((s < 0.0f) || (s > 1.0f) || (t < 0.0f) || (t > 1.0f)) return 0;
+    float pp[3], dp[3], pv[3];
+    for (int k = 0; k < 3; ++k) {
+        pp[k] = q->b[k] + q->e0[k] * s + q->e1[k] * t + q->n[k] * edge;
+        dp[k] = f[k] - pp[k];
+        pv[k] = prev[k] - f[k];
+    }
+    if (dot3(q->n, dp) > 0.0f) return 0;
+    const float nv = dot3(q->n, pv);
+    if (sqrtf(dot3(dp, dp)) > edge + nv) return 0;
+    int32_t mx = (int32_t)(t * (float)mw);
+    int32_t my = (int32_t)((float)q->mask_y + s * (float)q->mask_h);
+    mx = mx < 0 ? 0 : (mx >= (int32_t)mw ? (int32_t)mw - 1 : mx);
+    my = my < 0 ? 0 : (my >= (int32_t)mrows ? (int32_t)mrows - 1 : my);
+    if (mask[(size_t)my * mw + (size_t)mx] == 0u) return 0;
+    for (int k = 0; k < 3; ++k) f[k] = pp[k];
+    return 1;
+}
+
 void pbfo_delta(const pbfo_cfg *c, const float *x, const float *xs, const uint32_t *fr, float *delta)
 {
     pbfo_delta_hits(c, x, xs, fr, delta, NULL);
@@ -320,10 +384,13 @@ void pbfo_delta(const pbfo_cfg *c, const float *x, const float *xs, const uint32
 
 /* compute_delta.cl:44-186 (delta.w is left untouched, as in the reference).
  * hits (optional): per particle, which planes moved it (bit 2a: axis a low,
- * bit 2a+1: axis a high). */
+ * bit 2a+1: axis a high, bit 6+k: quad k). */
 void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const uint32_t *fr, float *delta,
-                     uint8_t *hits)
+                     uint32_t *hits)
 {
+    quadc qc[26];
+    const uint32_t nq = c->quads && c->mask ? (c->quad_count < 26 ? c->quad_count : 26) : 0;
+    for (uint32_t k = 0; k < nq; ++k) quad_consts(&c->quads[k], &qc[k]);
     const uint32_t n = c->n;
     const float h = c->h, h2 = h * h, rd = c->rest_density, K = c->st_k;
     float poly6, spiky;
@@ -369,21 +436,23 @@ void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const u
         }
         float fut[3];
         for (int k = 0; k < 3; ++k) fut[k] = pi[k] + (-spiky * sum[k]) / rd;
-        if (c->walls) {
+        uint32_t hb = 0;
+        if (c->walls || nq) {
             float ns[3];
             for (int k = 0; k < 3; ++k) ns[k] = pbfo_sinf(fut[k] * 5.0f / h);
-            /* compute_delta.cl:166 without its scene-scale "1.0 +" term */
+            /* compute_delta.cl:166; the walls leave out its scene-scale "1.0 +" */
             const float edge = (3.0f + ns[0] + ns[1] + ns[2]) * h * 0.03f;
             const float *prev = F4(x, i);
-            int hb = 0;
-            for (int a = 0; a < 3; ++a) {
-                hb |= wall_lo(fut, prev, a, c->box_lo[a], edge) << (2 * a);
-                hb |= wall_hi(fut, prev, a, c->box_hi[a], edge) << (2 * a + 1);
-            }
-            if (hits) hits[i] = (uint8_t)hb;
-        } else if (hits) {
-            hits[i] = 0;
+            if (c->walls)
+                for (int a = 0; a < 3; ++a) {
+                    hb |= (uint32_t)wall_lo(fut, prev, a, c->box_lo[a], edge) << (2 * a);
+                    hb |= (uint32_t)wall_hi(fut, prev, a, c->box_hi[a], edge) << (2 * a + 1);
+                }
+            const float eq = c->edge_base + edge;
+            for (uint32_t k = 0; k < nq; ++k)
+                hb |= (uint32_t)bounce_quad(&qc[k], prev, fut, eq, c->mask, c->mask_w, c->mask_rows) << (6 + k);
         }
+        if (hits) hits[i] = hb;
         float *o = F4(delta, i);
         for (int k = 0; k < 3; ++k) o[k] = fut[k] - pi[k];
     }
@@ -414,12 +483,6 @@ void pbfo_update_velocities(const pbfo_cfg *c, float *x, const float *xs, float 
     }
 }
 
-static inline void cross3(const float *a, const float *b, float *o)
-{
-    o[0] = a[1] * b[2] - a[2] * b[1];
-    o[1] = a[2] * b[0] - a[0] * b[2];
-    o[2] = a[0] * b[1] - a[1] * b[0];
-}
 
 /* apply_viscosity.cl:1-87 with Jacobi semantics: every v_j is read from the
  * pre-kernel snapshot v_in (the reference reads and writes one buffer, which
@@ -522,7 +585,7 @@ struct pbfo_sim {
     pbfo_cfg c;
     float *x, *v, *xs, *xs2, *rho, *lambda, *omega, *delta, *tmp4;
     uint32_t *keys, *keys_sorted, *perm, *cells, *friends;
-    uint8_t *wall_hits;  /* iters x n, written by each step when set */
+    uint32_t *wall_hits;  /* iters x n, written by each step when set */
     int sorted_once;
 };
 
@@ -599,7 +662,7 @@ void pbfo_step(pbfo_sim *s, uint32_t pause)
     pbfo_vorticity(c, s->xs, s->v, s->omega, s->friends);
 }
 
-void pbfo_record_walls(pbfo_sim *s, uint8_t *hits) { s->wall_hits = hits; }
+void pbfo_record_walls(pbfo_sim *s, uint32_t *hits) { s->wall_hits = hits; }
 
 void *pbfo_buffer(pbfo_sim *s, int which)
 {

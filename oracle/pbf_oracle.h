@@ -57,7 +57,22 @@ typedef struct pbfo_cfg {
     float box_lo[3];         /* Params.xMin,yMin,zMin: box walls                   */
     float box_hi[3];         /* Params.xMax,yMax,zMax                              */
     uint32_t walls;          /* 1: six box planes in the delta stage, 0: none      */
+    /* Collision quads (BouncePointQuad, compute_delta.cl:1-42, chained as
+     * :169-182), applied after the walls; quad_count <= 26. */
+    const struct pbfo_quad *quads;
+    uint32_t quad_count;
+    const uint32_t *mask;    /* mask_w x mask_rows texels; a quad moves a point  */
+    uint32_t mask_w;         /* only where its texel is nonzero (the red channel */
+    uint32_t mask_rows;      /* of Scene_fp_mask.png, Simulation.cpp:257-281)    */
+    float edge_base;         /* the "1.0 +" of compute_delta.cl:166 (the
+                                reference scene's scale); walls use none      */
 } pbfo_cfg;
+
+/* One collision quad: corner B, edges E0, E1 and its band of mask rows. */
+typedef struct pbfo_quad {
+    float b[3], e0[3], e1[3];
+    int32_t mask_y, mask_h;
+} pbfo_quad;
 
 /* Kernel normalisation constants as the reference bakes them into the
  * program: computed in double, printed with 6 significant digits, parsed as a
@@ -81,7 +96,7 @@ void pbfo_scaling(const pbfo_cfg *c, const float *xs, const uint32_t *friends, f
 void pbfo_pack(uint32_t n, const float *src4, const float *w, float *dst4);
 void pbfo_delta(const pbfo_cfg *c, const float *x, const float *xs, const uint32_t *friends, float *delta);
 void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const uint32_t *friends, float *delta,
-                     uint8_t *hits);
+                     uint32_t *hits);
 void pbfo_update_predicted(uint32_t n, const float *xs, const float *delta, float *xs_out);
 void pbfo_update_velocities(const pbfo_cfg *c, float *x, const float *xs, float *v);
 void pbfo_viscosity(const pbfo_cfg *c, const float *xs, const float *v_in, const uint32_t *friends,
@@ -101,9 +116,9 @@ void pbfo_step(pbfo_sim *s, uint32_t pause);
 /* which: 0 x, 1 v, 2 xs (w = rho), 3 rho, 4 lambda, 5 omega, 6 keys_sorted,
  * 7 perm, 8 cells, 9 friends, 10 keys (unsorted, this step) */
 void *pbfo_buffer(pbfo_sim *s, int which);
-/* Each later step writes, per solver iteration, which wall planes moved each
- * particle (iters x n bytes, bits as pbfo_delta_hits); NULL stops it. */
-void pbfo_record_walls(pbfo_sim *s, uint8_t *hits);
+/* Each later step writes, per solver iteration, which planes moved each
+ * particle (iters x n words, bits as pbfo_delta_hits); NULL stops it. */
+void pbfo_record_walls(pbfo_sim *s, uint32_t *hits);
 int pbfo_threads(void);
 
 #ifdef __cplusplus

@@ -35,6 +35,12 @@ class Cfg(ctypes.Structure):
         ("box_lo", ctypes.c_float * 3),
         ("box_hi", ctypes.c_float * 3),
         ("walls", ctypes.c_uint32),
+        ("quads", ctypes.c_void_p),
+        ("quad_count", ctypes.c_uint32),
+        ("mask", ctypes.c_void_p),
+        ("mask_w", ctypes.c_uint32),
+        ("mask_rows", ctypes.c_uint32),
+        ("edge_base", ctypes.c_float),
     ]
 
 
@@ -70,6 +76,7 @@ def lib() -> ctypes.CDLL:
             "pbfo_scaling": (None, [cp, vp, vp, vp, vp]),
             "pbfo_pack": (None, [ctypes.c_uint32, vp, vp, vp]),
             "pbfo_delta": (None, [cp, vp, vp, vp, vp]),
+            "pbfo_delta_hits": (None, [cp, vp, vp, vp, vp, vp]),
             "pbfo_update_predicted": (None, [ctypes.c_uint32, vp, vp, vp]),
             "pbfo_update_velocities": (None, [cp, vp, vp, vp]),
             "pbfo_viscosity": (None, [cp, vp, vp, vp, vp, vp]),
@@ -121,6 +128,35 @@ def cfg_from_params(p, options=None) -> Cfg:
     c.box_hi[0], c.box_hi[1], c.box_hi[2] = p.xMax, p.yMax, p.zMax
     c.walls = 1 if (options is None or options.walls) else 0
     return c
+
+
+class Quad(ctypes.Structure):
+    _fields_ = [
+        ("b", ctypes.c_float * 3),
+        ("e0", ctypes.c_float * 3),
+        ("e1", ctypes.c_float * 3),
+        ("mask_y", ctypes.c_int32),
+        ("mask_h", ctypes.c_int32),
+    ]
+
+
+def set_quads(cfg: Cfg, quads, mask=None, edge_base: float = 0.0) -> Cfg:
+    """Attaches collision quads ((corner, e0, e1, mask_y, mask_h) tuples) and
+    a (rows, width) mask (None: one solid texel) to cfg; cfg keeps them
+    alive."""
+    arr = (Quad * max(1, len(quads)))()
+    for k, (b, e0, e1, my, mh) in enumerate(quads):
+        for a in range(3):
+            arr[k].b[a], arr[k].e0[a], arr[k].e1[a] = b[a], e0[a], e1[a]
+        arr[k].mask_y, arr[k].mask_h = int(my), int(mh)
+    m = np.ones((1, 1), np.uint32) if mask is None else np.ascontiguousarray(mask, np.uint32)
+    cfg.quads = ctypes.cast(arr, ctypes.c_void_p)
+    cfg.quad_count = len(quads)
+    cfg.mask = m.ctypes.data
+    cfg.mask_rows, cfg.mask_w = m.shape
+    cfg.edge_base = edge_base
+    cfg._keep_quads = (arr, m)
+    return cfg
 
 
 class OracleSim:
@@ -216,11 +252,11 @@ class FrozenRollout:
                               np.float32)
         sim = OracleSim(cfg)
         sim.set_state(x4, v4)
-        hits = np.zeros((iters, n), np.uint8)
+        hits = np.zeros((iters, n), np.uint32)
         lib().pbfo_record_walls(sim._s, ptr(hits))
         self.perm = np.zeros((steps, n), np.uint32)
         self.friends = np.zeros((steps, n * CIRCLES * (1 + PER_CIRCLE)), np.uint32)
-        self.hits = np.zeros((steps, iters, n), np.uint8)
+        self.hits = np.zeros((steps, iters, n), np.uint32)
         ids = np.arange(n)
         for s in range(steps):
             sim.step()

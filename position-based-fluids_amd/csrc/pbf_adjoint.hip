@@ -152,18 +152,32 @@ __device__ __forceinline__ void block_max(float v, float* out) {
     }
 }
 
-// Transpose of the walls (compute_delta.cl:1-42 as applied in apply_walls):
-// a coordinate a plane moved becomes lo + edge(F) or hi - edge(F), with
-// edge(F) = (3 + sum_c sin(5 F_c / h)) 0.03 h; the others pass through.
+// Transpose of the boundaries (compute_delta.cl:1-42,163-182 as applied in
+// apply_walls), in reverse order.  A quad k that moved the point maps the
+// post-wall position F' to B + E0 s + E1 t + n (edge_base + edge(F)), with
+// s, t the linear parameters of F' (so g passes through the projection's
+// transpose, and n.g goes to the edge); a coordinate a plane moved becomes
+// lo + edge(F) or hi - edge(F), with edge(F) = (3 + sum_c sin(5 F_c / h))
+// 0.03 h; the others pass through.
 __device__ __forceinline__ V3 wall_vjp(const float4& fut, V3 g, const SceneConsts& c) {
-    if (!c.walls) return g;
     const uint32_t hits = __float_as_uint(fut.w);
     if (!hits) return g;
+    float eg = 0.0f;   // dL/d edge through the quads
+    for (uint32_t k = c.quad_count; k-- > 0;) {
+        if (!((hits >> (6 + k)) & 1u)) continue;
+        const QuadConsts& q = c.quads[k];
+        const float gs = q.e00 * g.x + q.e01 * g.y + q.e02 * g.z;
+        const float gt = q.e10 * g.x + q.e11 * g.y + q.e12 * g.z;
+        eg += q.n0 * g.x + q.n1 * g.y + q.n2 * g.z;
+        const float ge = q.invdet * (q.bb * gs - q.aa * gt);   // dL/de, e = E1.(B - F')
+        const float gd = q.invdet * (q.bb * gt - q.cc * gs);   // dL/dd, d = E0.(B - F')
+        g = {-(q.e10 * ge + q.e00 * gd), -(q.e11 * ge + q.e01 * gd), -(q.e12 * ge + q.e02 * gd)};
+    }
     const float sx = (hits & 2u) ? -1.0f : ((hits & 1u) ? 1.0f : 0.0f);
     const float sy = (hits & 8u) ? -1.0f : ((hits & 4u) ? 1.0f : 0.0f);
     const float sz = (hits & 32u) ? -1.0f : ((hits & 16u) ? 1.0f : 0.0f);
     V3 o = {sx != 0.0f ? 0.0f : g.x, sy != 0.0f ? 0.0f : g.y, sz != 0.0f ? 0.0f : g.z};
-    const float e = sx * g.x + sy * g.y + sz * g.z;
+    const float e = sx * g.x + sy * g.y + sz * g.z + eg;
     const float k = 0.15f * e;   // d edge / dF_c = 0.03 h * 5/h * cos(5 F_c / h)
     o.x += k * cosf(fut.x * 5.0f / c.h);
     o.y += k * cosf(fut.y * 5.0f / c.h);

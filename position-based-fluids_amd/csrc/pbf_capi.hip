@@ -102,7 +102,7 @@ void free_all(pbf_ctx* ctx) {
     void* ptrs[] = {ctx->x, ctx->x_alt, ctx->xs[0], ctx->xs[1], ctx->v, ctx->v_alt, ctx->omega, ctx->rho,
                     ctx->lambda, ctx->keys_in, ctx->keys_tmp, ctx->keys_sorted, ctx->vals_a, ctx->vals_b,
                     ctx->perm, ctx->sort.ws, ctx->cells, ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->err, ctx->sc_d,
-                    ctx->bt_predict.d};
+                    ctx->bt_predict.d, ctx->geom_d};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& t : ctx->bt)
@@ -431,6 +431,77 @@ void collect_timings(pbf_ctx* ctx) {
     ctx->events_used = 0;
 }
 
+// BouncePointQuad's per-quad constants (compute_delta.cl:3-9) in binary32,
+// dot products summed left to right; normalize() as v * (1 / |v|).
+QuadConsts quad_consts(const pbf_quad& q) {
+    QuadConsts o{};
+    o.b0 = q.origin[0]; o.b1 = q.origin[1]; o.b2 = q.origin[2];
+    o.e00 = q.e0[0]; o.e01 = q.e0[1]; o.e02 = q.e0[2];
+    o.e10 = q.e1[0]; o.e11 = q.e1[1]; o.e12 = q.e1[2];
+    o.aa = o.e00 * o.e00 + o.e01 * o.e01 + o.e02 * o.e02;
+    o.bb = o.e00 * o.e10 + o.e01 * o.e11 + o.e02 * o.e12;
+    o.cc = o.e10 * o.e10 + o.e11 * o.e11 + o.e12 * o.e12;
+    o.invdet = 1.0f / (o.aa * o.cc - o.bb * o.bb);
+    const float cx = o.e01 * o.e12 - o.e02 * o.e11;
+    const float cy = o.e02 * o.e10 - o.e00 * o.e12;
+    const float cz = o.e00 * o.e11 - o.e01 * o.e10;
+    const float il = 1.0f / sqrtf(cx * cx + cy * cy + cz * cz);
+    o.n0 = cx * il; o.n1 = cy * il; o.n2 = cz * il;
+    o.mask_y = q.mask_y;
+    o.mask_h = q.mask_h;
+    return o;
+}
+
+// Replaces scene `scene`'s quads and mask with q, m (swapped in on
+// success): one device buffer then holds every scene's quads and mask bits,
+// and the scene constants point into it.  On failure the context is left as
+// it was.  The caller has drained the stream.
+int upload_geometry(pbf_ctx* ctx, uint32_t scene, std::vector<QuadConsts>& q, std::vector<uint32_t>& m,
+                    uint32_t mask_w, uint32_t mask_rows, float edge_base) {
+    ctx->quads.resize(ctx->S);
+    ctx->mask_bits.resize(ctx->S);
+    auto quads_of = [&](uint32_t s) -> const std::vector<QuadConsts>& { return s == scene ? q : ctx->quads[s]; };
+    auto mask_of = [&](uint32_t s) -> const std::vector<uint32_t>& { return s == scene ? m : ctx->mask_bits[s]; };
+    size_t bytes = 0;
+    for (uint32_t s = 0; s < ctx->S; ++s) {
+        bytes = (bytes + quads_of(s).size() * sizeof(QuadConsts) + 255) & ~size_t(255);
+        bytes = (bytes + mask_of(s).size() * 4 + 255) & ~size_t(255);
+    }
+    char* d = nullptr;
+    if (bytes && hipMalloc(&d, bytes) != hipSuccess)
+        return fail(ctx, PBF_E_NOMEM, "collision quads: hipMalloc(%zu) failed", bytes);
+    std::vector<SceneConsts> sc = ctx->sc;
+    sc[scene].quad_count = (uint32_t)q.size();
+    sc[scene].mask_w = q.empty() ? 0u : mask_w;
+    sc[scene].mask_rows = q.empty() ? 0u : mask_rows;
+    sc[scene].edge_base = q.empty() ? 0.0f : edge_base;
+    size_t off = 0;
+    hipError_t e = hipSuccess;
+    for (uint32_t s = 0; s < ctx->S && e == hipSuccess; ++s) {
+        const auto& qs = quads_of(s);
+        const auto& ms = mask_of(s);
+        sc[s].quads = qs.empty() ? nullptr : reinterpret_cast<const QuadConsts*>(d + off);
+        if (!qs.empty()) e = hipMemcpy(d + off, qs.data(), qs.size() * sizeof(QuadConsts), hipMemcpyHostToDevice);
+        off = (off + qs.size() * sizeof(QuadConsts) + 255) & ~size_t(255);
+        sc[s].mask = ms.empty() ? nullptr : reinterpret_cast<const uint32_t*>(d + off);
+        if (!ms.empty() && e == hipSuccess) e = hipMemcpy(d + off, ms.data(), ms.size() * 4, hipMemcpyHostToDevice);
+        off = (off + ms.size() * 4 + 255) & ~size_t(255);
+    }
+    if (e == hipSuccess) e = hipMemcpy(ctx->sc_d, sc.data(), sizeof(SceneConsts) * ctx->S, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (d) (void)hipFree(d);
+        // the old table is still valid: restore the device copy of the constants
+        (void)hipMemcpy(ctx->sc_d, ctx->sc.data(), sizeof(SceneConsts) * ctx->S, hipMemcpyHostToDevice);
+        return fail(ctx, PBF_E_HIP, "collision quads: %s", hipGetErrorString(e));
+    }
+    if (ctx->geom_d) (void)hipFree(ctx->geom_d);
+    ctx->geom_d = d;
+    ctx->sc.swap(sc);
+    ctx->quads[scene].swap(q);
+    ctx->mask_bits[scene].swap(m);
+    return PBF_OK;
+}
+
 }  // namespace pbf
 
 extern "C" {
@@ -498,6 +569,41 @@ int pbf_set_state(pbf_ctx* ctx, const float* x4, const float* v4, uint32_t n) {
     ctx->have_state = true;
     ctx->order_sorted = false;   // cells still hold the old keys until reset
     return PBF_OK;
+}
+
+int pbf_set_collision_quads(pbf_ctx* ctx, uint32_t scene, const pbf_quad* quads, uint32_t count, const uint32_t* mask,
+                            uint32_t mask_w, uint32_t mask_rows, float edge_base) {
+    if (!ctx) return PBF_E_INVALID;
+    if (scene >= ctx->S) return fail(ctx, PBF_E_INVALID, "scene %u out of range (%u scenes)", scene, ctx->S);
+    if (count > PBF_MAX_QUADS) return fail(ctx, PBF_E_INVALID, "%u quads > %d", count, PBF_MAX_QUADS);
+    std::vector<pbf::QuadConsts> qc;
+    std::vector<uint32_t> bits;
+    if (count) {
+        if (!quads || !mask) return fail(ctx, PBF_E_INVALID, "quads or mask is NULL");
+        if (mask_w == 0 || mask_rows == 0 || (uint64_t)mask_w * mask_rows > (1ull << 31))
+            return fail(ctx, PBF_E_INVALID, "mask must have 1..2^31 texels (got %u x %u)", mask_w, mask_rows);
+        if (!isfinite(edge_base)) return fail(ctx, PBF_E_INVALID, "edge_base must be finite");
+        for (uint32_t k = 0; k < count; ++k) {
+            const pbf_quad& q = quads[k];
+            for (int a = 0; a < 3; ++a)
+                if (!isfinite(q.origin[a]) || !isfinite(q.e0[a]) || !isfinite(q.e1[a]))
+                    return fail(ctx, PBF_E_INVALID, "quad %u: non-finite corner or edge", k);
+            if (q.mask_y < -(1 << 24) || q.mask_y > (1 << 24) || q.mask_h < -(1 << 24) || q.mask_h > (1 << 24))
+                return fail(ctx, PBF_E_INVALID, "quad %u: mask_y and mask_h must be within +-2^24", k);
+            const pbf::QuadConsts c = pbf::quad_consts(q);
+            if (!(c.aa * c.cc - c.bb * c.bb > 0.0f) || !isfinite(c.invdet) || !isfinite(c.n0) || !isfinite(c.n1) ||
+                !isfinite(c.n2))
+                return fail(ctx, PBF_E_INVALID, "quad %u: its edges are parallel or degenerate", k);
+            qc.push_back(c);
+        }
+        const uint32_t stride = (mask_w + 31) / 32;
+        bits.assign((size_t)stride * mask_rows, 0u);
+        for (uint32_t y = 0; y < mask_rows; ++y)
+            for (uint32_t x = 0; x < mask_w; ++x)
+                if (mask[(size_t)y * mask_w + x]) bits[(size_t)y * stride + (x >> 5)] |= 1u << (x & 31);
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return pbf::upload_geometry(ctx, scene, qc, bits, mask_w, mask_rows, edge_base);
 }
 
 int pbf_step(pbf_ctx* ctx, uint32_t pause) {

@@ -14,6 +14,21 @@ constexpr int kCircles = 5;          // Params.friendsCircles (src/ParamUtils.cp
 constexpr int kPerCircle = 50;       // Params.particlesPerCircle (src/ParamUtils.cpp:99)
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // END_OF_CELL_LIST (src/Simulation.cpp:170)
 
+constexpr uint32_t kMaxQuads = 26;   // hit bits 6..31 of a particle's boundary record
+
+// One collision quad with the constants BouncePointQuad derives from it
+// (compute_delta.cl:3-9), computed once on the host in binary32: corner B,
+// edges E0, E1, unit normal n = cross(E0, E1) / |cross(E0, E1)|, the Gram
+// entries aa = E0.E0, bb = E0.E1, cc = E1.E1 and invdet = 1 / (aa cc - bb^2);
+// mask rows [mask_y, mask_y + mask_h).
+struct QuadConsts {
+    float b0, b1, b2, e00, e01, e02, e10, e11, e12, n0, n1, n2;
+    float aa, bb, cc, invdet;
+    int32_t mask_y, mask_h;
+    uint32_t pad0, pad1;
+};
+static_assert(sizeof(QuadConsts) == 80, "QuadConsts is 20 words");
+
 // Constants of one scene.  Kernels read them through a workgroup-uniform
 // index (the block table), so they land in scalar registers.
 struct SceneConsts {
@@ -28,8 +43,12 @@ struct SceneConsts {
     uint32_t cell_base;             // first global cell id of this scene
     uint32_t first, count;          // particle range of this scene
     float circ_t[4];                // r^2 at which the friend circle reaches 1..4
+    const QuadConsts* quads;        // collision quads (device), quad_count of them
+    const uint32_t* mask;           // quad mask, 1 bit per texel, rows of (mask_w + 31) / 32 words
+    uint32_t quad_count, mask_w, mask_rows;
+    float edge_base;                // added to the noise edge for the quads
 };
-static_assert(sizeof(SceneConsts) == 128, "SceneConsts is 32 words");
+static_assert(sizeof(SceneConsts) == 160, "SceneConsts is 40 words");
 
 // utilities.cl:69-77 on the 32-bit two's-complement pattern.
 __device__ __forceinline__ uint32_t expand_bits(int32_t xi) {
@@ -244,20 +263,63 @@ __device__ __forceinline__ float wall_edge(float fx, float fy, float fz, const S
     return (3.0f + nx + ny + nz) * c.h * 0.03f;
 }
 
-// Planes in the order x-lo, x-hi, y-lo, y-hi, z-lo, z-hi.  Returns which
-// moved the point: bit 2a (lo) and 2a+1 (hi) of axis a.
+// BouncePointQuad (compute_delta.cl:1-42): a point inside the quad's
+// parallelogram (s, t in [0, 1]), behind its plane offset by `edge` along n
+// and within the force zone (|F - plane point| <= edge + n.(prev - F)) moves
+// onto the offset plane where the mask texel ((int)(t W), mask_y +
+// (int)(s mask_h)), clamped to the image as CLK_ADDRESS_CLAMP_TO_EDGE
+// (utilities.cl:89), is set.  Returns whether it moved the point.
+__device__ __forceinline__ bool bounce_quad(const QuadConsts& q, const float4& prev, float& fx, float& fy, float& fz,
+                                            float edge, const SceneConsts& c) {
+    const float Dx = q.b0 - fx, Dy = q.b1 - fy, Dz = q.b2 - fz;
+    const float e = q.e10 * Dx + q.e11 * Dy + q.e12 * Dz;
+    const float d = q.e00 * Dx + q.e01 * Dy + q.e02 * Dz;
+    const float s = q.invdet * (q.bb * e - q.cc * d);
+    const float t = q.invdet * (q.bb * d - q.aa * e);
+    if ((s < 0.0f) || (s > 1.0f) || (t < 0.0f) || (t > 1.0f)) return false;
+    const float px = q.b0 + q.e00 * s + q.e10 * t + q.n0 * edge;
+    const float py = q.b1 + q.e01 * s + q.e11 * t + q.n1 * edge;
+    const float pz = q.b2 + q.e02 * s + q.e12 * t + q.n2 * edge;
+    const float dx = fx - px, dy = fy - py, dz = fz - pz;
+    if (q.n0 * dx + q.n1 * dy + q.n2 * dz > 0.0f) return false;
+    const float nv = q.n0 * (prev.x - fx) + q.n1 * (prev.y - fy) + q.n2 * (prev.z - fz);
+    if (sqrtf(dx * dx + dy * dy + dz * dz) > edge + nv) return false;
+    int32_t mx = (int32_t)(t * (float)c.mask_w);
+    int32_t my = (int32_t)((float)q.mask_y + s * (float)q.mask_h);
+    mx = mx < 0 ? 0 : (mx >= (int32_t)c.mask_w ? (int32_t)c.mask_w - 1 : mx);
+    my = my < 0 ? 0 : (my >= (int32_t)c.mask_rows ? (int32_t)c.mask_rows - 1 : my);
+    const uint32_t stride = (c.mask_w + 31u) >> 5;
+    if (!((c.mask[(size_t)my * stride + ((uint32_t)mx >> 5)] >> ((uint32_t)mx & 31u)) & 1u)) return false;
+    fx = px;
+    fy = py;
+    fz = pz;
+    return true;
+}
+
+// The delta stage's boundaries (compute_delta.cl:163-182): the force-zone
+// edge from the unconstrained position, the box planes in the order x-lo,
+// x-hi, y-lo, y-hi, z-lo, z-hi, then the collision quads in order with edge
+// edge_base + edge.  Returns which moved the point: bit 2a (lo) and 2a+1 (hi)
+// of axis a, bit 6+k for quad k.
 __device__ __forceinline__ uint32_t apply_walls(float& fx, float& fy, float& fz, const float4& prev,
                                                 const SceneConsts& c) {
     const float edge = wall_edge(fx, fy, fz, c);
-    bool h0 = false, h1 = false, h2 = false, h3 = false, h4 = false, h5 = false;
-    fx = wall_lo(fx, prev.x, c.lo0, edge, h0);
-    fx = wall_hi(fx, prev.x, c.hi0, edge, h1);
-    fy = wall_lo(fy, prev.y, c.lo1, edge, h2);
-    fy = wall_hi(fy, prev.y, c.hi1, edge, h3);
-    fz = wall_lo(fz, prev.z, c.lo2, edge, h4);
-    fz = wall_hi(fz, prev.z, c.hi2, edge, h5);
-    return (uint32_t)h0 | ((uint32_t)h1 << 1) | ((uint32_t)h2 << 2) | ((uint32_t)h3 << 3) | ((uint32_t)h4 << 4) |
-           ((uint32_t)h5 << 5);
+    uint32_t hits = 0;
+    if (c.walls) {
+        bool h0 = false, h1 = false, h2 = false, h3 = false, h4 = false, h5 = false;
+        fx = wall_lo(fx, prev.x, c.lo0, edge, h0);
+        fx = wall_hi(fx, prev.x, c.hi0, edge, h1);
+        fy = wall_lo(fy, prev.y, c.lo1, edge, h2);
+        fy = wall_hi(fy, prev.y, c.hi1, edge, h3);
+        fz = wall_lo(fz, prev.z, c.lo2, edge, h4);
+        fz = wall_hi(fz, prev.z, c.hi2, edge, h5);
+        hits = (uint32_t)h0 | ((uint32_t)h1 << 1) | ((uint32_t)h2 << 2) | ((uint32_t)h3 << 3) |
+                    ((uint32_t)h4 << 4) | ((uint32_t)h5 << 5);
+    }
+    const float eq = c.edge_base + edge;
+    for (uint32_t k = 0; k < c.quad_count; ++k)
+        if (bounce_quad(c.quads[k], prev, fx, fy, fz, eq, c)) hits |= 1u << (6 + k);
+    return hits;
 }
 
 }  // namespace pbf

@@ -581,8 +581,8 @@ struct DeltaAcc {
     }
 };
 
-// TAPE also stores the position before the walls and which planes moved it
-// (bits in w) for the adjoint (fut).
+// TAPE also stores the position before the walls and which planes and quads
+// moved it (bits in w, as apply_walls returns them) for the adjoint (fut).
 template <int BS, bool LAST, bool TAPE>
 __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
                                              const float4* __restrict__ xs, float4* __restrict__ xs_next,
@@ -603,7 +603,7 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
     const float4 prev = x[i];
     const float ux = fx, uy = fy, uz = fz;
     uint32_t hits = 0;
-    if (c.walls) hits = apply_walls(fx, fy, fz, prev, c);
+    if (c.walls | c.quad_count) hits = apply_walls(fx, fy, fz, prev, c);
     if (TAPE) fut[i] = make_float4(ux, uy, uz, __uint_as_float(hits));
     const float dx = fx - p.x, dy = fy - p.y, dz = fz - p.z;
     float4 o;

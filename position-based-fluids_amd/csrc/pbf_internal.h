@@ -122,6 +122,13 @@ struct pbf_ctx {
 
     uint32_t* err = nullptr;
 
+    // collision quads (pbf_set_collision_quads), per scene on the host; one
+    // device buffer holds every scene's QuadConsts and mask bits, and the
+    // SceneConsts point into it
+    std::vector<std::vector<pbf::QuadConsts>> quads;
+    std::vector<std::vector<uint32_t>> mask_bits;
+    void* geom_d = nullptr;
+
     uint32_t pause_now = 0;     // pause flag of the step being enqueued
     bool have_state = false;
     bool cells_hold_keys = false;  // cells hold the entries of keys_sorted (reset them first)

@@ -78,6 +78,20 @@ class PbfOptions(ctypes.Structure):
     ]
 
 
+class PbfQuad(ctypes.Structure):
+    """pbf_quad: corner, edges E0, E1 and the quad's band of mask rows."""
+    _fields_ = [
+        ("origin", ctypes.c_float * 3),
+        ("e0", ctypes.c_float * 3),
+        ("e1", ctypes.c_float * 3),
+        ("mask_y", ctypes.c_int32),
+        ("mask_h", ctypes.c_int32),
+    ]
+
+
+PBF_MAX_QUADS = 26
+
+
 class PbfTiming(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 48),
@@ -155,6 +169,8 @@ _SIGNATURES = {
     "pbf_friend_circle_thresholds": (ctypes.c_int, [ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "pbf_copy_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "pbf_set_state": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32]),
+    "pbf_set_collision_quads": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(PbfQuad), ctypes.c_uint32, _UP,
+                                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]),
     "pbf_step": (ctypes.c_int, [_VP, ctypes.c_uint32]),
     "pbf_run": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32]),
     "pbf_sync": (ctypes.c_int, [_VP]),

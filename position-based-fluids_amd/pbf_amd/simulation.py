@@ -61,6 +61,24 @@ class Simulation:
         vv = None if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 4)
         check(lib().pbf_set_state(self._ctx, _ptr(x), _ptr(vv), x.shape[0]), self._ctx)
 
+    def set_collision_quads(self, quads, mask=None, edge_base: float = 0.0, scene: int = 0):
+        """pbf_set_collision_quads: the scene's collision quads (PbfQuad or
+        (origin, e0, e1, mask_y, mask_h) each), applied after the walls, and
+        their mask as a (rows, width) texel array (nonzero: solid; None: one
+        solid texel, so every quad is solid everywhere).  No quads removes
+        them."""
+        from .boundaries import quad_array
+        self._ensure()
+        arr, k = quad_array(quads)
+        mp, w, rows = None, 0, 0
+        if k:
+            m = np.ones((1, 1), np.uint32) if mask is None else np.ascontiguousarray(mask, np.uint32)
+            if m.ndim != 2:
+                raise ValueError("mask must be a (rows, width) array")
+            rows, w = m.shape
+            mp = m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        check(lib().pbf_set_collision_quads(self._ctx, int(scene), arr, k, mp, w, rows, float(edge_base)), self._ctx)
+
     def Step(self):
         """Simulation::Step (src/Simulation.cpp:554-621), asynchronous."""
         self._ensure()

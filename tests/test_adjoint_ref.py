@@ -23,6 +23,24 @@ def frozen(oracle):
     return oracle.FrozenRollout(cfg, scene.x, scene.v, 8)
 
 
+# a ramp over the floor (normal up and towards -x) and a plate facing -x at
+# the far side of the block, both with a half-solid random mask
+QUADS = [((0.0, 0.03, 0.0), (0.0, 0.0, 0.8), (0.8, 0.1, 0.0), 0, 16),
+         ((0.38, 0.0, 0.0), (0.0, 0.0, 0.8), (0.0, 0.8, 0.0), 16, 16)]
+
+
+def quad_mask(seed=0):
+    return (np.random.default_rng(seed).random((32, 16)) < 0.5).astype(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def frozen_quads(oracle):
+    scene = pbf_amd.scenes.inverse_problem(8, 8, 8, seed=3)
+    cfg = oracle.cfg_from_params(scene.params, scene.options)
+    oracle.set_quads(cfg, QUADS, quad_mask(), 0.004)
+    return oracle.FrozenRollout(cfg, scene.x, scene.v, 8)
+
+
 def _direction(fr, what, seed):
     rng = np.random.default_rng(seed)
     dx = dv = dc = None
@@ -94,3 +112,44 @@ def test_frozen_structure_records_the_walls(frozen):
     hi = frozen.hits & 0b101010
     assert lo.any()
     assert not hi.any()
+
+
+def test_frozen_structure_records_the_quads(frozen_quads):
+    q = frozen_quads.hits >> 6
+    assert (q & 1).any() and (q & 2).any()
+    assert not (q >> 2).any()
+
+
+def test_binary32_instantiation_with_quads_is_the_oracle(frozen_quads):
+    x32, v32, _, _ = frozen_quads.jvp32()
+    assert np.array_equal(x32, frozen_quads.xT32)
+    assert np.array_equal(v32, frozen_quads.vT32)
+
+
+@pytest.mark.parametrize("what", ["v0", "gravity"])
+def test_fp64_jvp_with_quads_equals_central_differences(frozen_quads, what):
+    fr = frozen_quads
+    dx, dv, dc = _direction(fr, what, 3)
+    tgt = fr.xT32 + np.random.default_rng(1).normal(0, 0.01, fr.xT32.shape)
+
+    def loss(xT):
+        return np.mean(np.sum((xT - tgt) ** 2, axis=1))
+
+    xb, _, dxT, _ = fr.jvp64(dx0=dx, dv0=dv, dcoef=dc)
+    jvp = np.sum(2.0 * (xb - tgt) * dxT) / fr.n
+    x0, v0, c0 = fr.x0.astype(np.float64), fr.v0.astype(np.float64), fr.coef0.astype(np.float64)
+    e = 1e-6
+
+    def L(k):
+        return loss(fr.rollout64(x0, v0 + (0 if dv is None else k * e * dv), c0 + (0 if dc is None else k * e * dc))[0])
+
+    fd = (L(-2.0) - 8.0 * L(-1.0) + 8.0 * L(1.0) - L(2.0)) / (12.0 * e)
+    assert abs(fd - jvp) <= 1e-5 * abs(jvp), (fd, jvp)
+
+
+def test_binary32_jvp_with_quads_equals_chained_fp64_differences(frozen_quads):
+    _, dv, _ = _direction(frozen_quads, "v0", 4)
+    _, _, dx32, dv32 = frozen_quads.jvp32(dv0=dv)
+    dxf, dvf = frozen_quads.tangent_fd(dv0=dv, delta=1e-6)
+    assert np.abs(dx32 - dxf).max() <= 1e-4 * np.abs(dx32).max()
+    assert np.abs(dv32 - dvf).max() <= 1e-4 * np.abs(dv32).max()

@@ -39,12 +39,15 @@ def _dev(a):
 class Problem:
     """A rollout on the GPU, its loss gradient, and the frozen checker."""
 
-    def __init__(self, oracle, scene, steps, interval, seed=0):
+    def __init__(self, oracle, scene, steps, interval, seed=0, quads=None):
         self.scene = scene
         self.steps = steps
         self.sim = pbf_amd.Simulation(scene.params, scene.options)
-        self.roll = Rollout(self.sim, steps, interval)
         cfg = oracle.cfg_from_params(scene.params, scene.options)
+        if quads is not None:
+            self.sim.set_collision_quads(*quads)
+            oracle.set_quads(cfg, *quads)
+        self.roll = Rollout(self.sim, steps, interval)
         self.fr = oracle.FrozenRollout(cfg, scene.x, scene.v, steps)
         n = scene.n
         xT, vT = self.roll.forward_raw(_dev(scene.x), _dev(scene.v))
@@ -134,6 +137,35 @@ def test_short_horizon_matches_fp64_central_differences(oracle):
             xm, vm = p.fr.rollout64(v0=v0 - (0 if ddv is None else e * ddv), coef=c0 - (0 if ddc is None else e * ddc))
             fd = (loss(xp, vp) - loss(xm, vm)) / (2 * e)
             assert abs(adj - fd) <= RTOL * abs(fd), (adj, fd)
+    finally:
+        p.close()
+
+
+def test_gradients_through_collision_quads(oracle):
+    # the ramp and the plate move points in most substeps; the adjoint passes
+    # through the projections they make
+    from _quads import box_quads
+    scene = _scene(seed=8)
+    quads, mask = box_quads(scene.params, plate_x=0.5)
+    p = Problem(oracle, scene, steps=16, interval=4, seed=2, quads=(quads, mask, 0.003))
+    try:
+        assert ((p.fr.hits >> 6) & 1).any() and ((p.fr.hits >> 6) & 2).any()
+        assert np.array_equal(p.xT, p.fr.xT32)
+        rng = np.random.default_rng(12)
+        dv = rng.normal(0, 1, (p.scene.n, 3))
+        adj = float(np.sum(p.gv0[:, :3] * dv))
+        _, _, dxT, dvT = p.fr.jvp32(dv0=dv)
+        jvp = p.dL(dxT, dvT)
+        assert abs(adj - jvp) <= RTOL * abs(jvp), (adj, jvp)
+        fx, fv = p.fr.tangent_fd(dv0=dv)
+        fd = p.dL(fx, fv)
+        assert abs(adj - fd) <= RTOL * abs(fd), (adj, fd)
+        for k in range(5):
+            dc = np.zeros(5)
+            dc[k] = 1.0
+            _, _, dxT, dvT = p.fr.jvp32(dcoef=dc)
+            jvp = p.dL(dxT, dvT)
+            assert abs(p.gc[0, k] - jvp) <= RTOL * abs(jvp), (k, p.gc[0, k], jvp)
     finally:
         p.close()
 

@@ -25,8 +25,13 @@ def scene_slices(structure, offsets, cell_bases, s, gbs):
     return keys, perm, cells, counts, idx
 
 
-def check_batch(scenes, steps, graph=False):
+def check_batch(scenes, steps, graph=False, quads=None):
+    """quads: per scene None or (quads, mask, edge_base)."""
+    quads = quads or [None] * len(scenes)
     sim = pbf_amd.BatchSimulation([s.params for s in scenes], [s.options for s in scenes])
+    for s, q in enumerate(quads):
+        if q is not None:
+            sim.set_collision_quads(q[0], q[1], q[2], scene=s)
     sim.set_state(np.concatenate([s.x for s in scenes]), np.concatenate([s.v for s in scenes]))
     if graph:
         sim.set_graph(True)
@@ -36,7 +41,10 @@ def check_batch(scenes, steps, graph=False):
     off = sim.offsets
     cell_bases = np.concatenate([[0], np.cumsum([int(s.params.gridBufSize) for s in scenes])])
     for s, scene in enumerate(scenes):
-        ref = pbf_oracle.OracleSim(pbf_oracle.cfg_from_params(scene.params, scene.options))
+        cfg = pbf_oracle.cfg_from_params(scene.params, scene.options)
+        if quads[s] is not None:
+            pbf_oracle.set_quads(cfg, *quads[s])
+        ref = pbf_oracle.OracleSim(cfg)
         ref.set_state(scene.x, scene.v)
         for _ in range(steps):
             ref.step()
@@ -72,3 +80,12 @@ def test_batch_per_scene_gravity():
     for s, g in zip(scenes, ((0.0, -9.8, 0.0), (3.0, -9.0, 1.5), (-2.0, -7.0, -6.2))):
         s.options = pbf_amd.default_options(gravity=g)
     check_batch(scenes, 2)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_per_scene_quads(graph):
+    from _quads import box_quads
+    scenes = pbf_amd.ensemble(3, seed=4, n_min=1000, n_max=3000)
+    q0, m0 = box_quads(scenes[0].params, mask_seed=1)
+    q2, m2 = box_quads(scenes[2].params, plate_x=0.3, mask_seed=2)
+    check_batch(scenes, 6, graph=graph, quads=[(q0, m0, 0.002), None, (q2[1:], m2, 0.0)])

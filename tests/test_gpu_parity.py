@@ -153,6 +153,84 @@ def test_friend_overflow_flag():
     sim.close()
 
 
+def test_coincident_particles():
+    # pairs of particles at exactly one point: finite (no 0/0) and bitwise
+    # equal to the oracle
+    scene = pbf_amd.config("dam4k", seed=11)
+    x = scene.x.copy()
+    x[1::2, :3] = x[0::2, :3]
+    scene = pbf_amd.Scene(scene.params, scene.options, x, np.zeros_like(x))
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 3)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    assert np.isfinite(sim.get_state()["x"]).all()
+    sim.close()
+
+
+def test_collision_quads():
+    # a ramp over the floor and a plate in the path of the dam break, with a
+    # random mask: bitwise equal to the oracle, and the quads do move points
+    from _quads import box_quads
+    scene = pbf_amd.config("dam4k", seed=12)
+    quads, mask = box_quads(scene.params)
+    sim = pbf_amd.Simulation(scene.params, scene.options)
+    sim.set_collision_quads(quads, mask, 0.003)
+    sim.set_state(scene.x, scene.v)
+    cfg = pbf_oracle.set_quads(pbf_oracle.cfg_from_params(scene.params, scene.options), quads, mask, 0.003)
+    ref = pbf_oracle.OracleSim(cfg)
+    ref.set_state(scene.x, scene.v)
+    advance(sim, ref, 24)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    plain = pbf_oracle.OracleSim(pbf_oracle.cfg_from_params(scene.params, scene.options))
+    plain.set_state(scene.x, scene.v)
+    for _ in range(24):
+        plain.step()
+    assert not np.array_equal(plain.get_state()["x"], ref.get_state()["x"])
+    # removing the quads restores the plain substep
+    sim.set_collision_quads([])
+    sim.set_state(scene.x, scene.v)
+    sim.run(24)
+    assert np.array_equal(sim.get_state()["x"], plain.get_state()["x"])
+    sim.close()
+
+
+def test_collision_quads_graph_replay():
+    from _quads import box_quads
+    scene = pbf_amd.config("dam4k", seed=13)
+    quads, mask = box_quads(scene.params, plate_x=0.4)
+    out = []
+    for graph in (False, True):
+        sim = pbf_amd.Simulation(scene.params, scene.options)
+        sim.set_graph(graph)
+        sim.set_state(scene.x, scene.v)
+        sim.run(2)
+        sim.set_collision_quads(quads, mask)   # after capture: the graph reads the new table
+        sim.run(10)
+        out.append(sim.get_state()["x"])
+        sim.close()
+    assert np.array_equal(out[0], out[1])
+
+
+def test_collision_quads_are_validated():
+    scene = pbf_amd.config("dam4k", seed=1)
+    sim = pbf_amd.Simulation(scene.params, scene.options)
+    ok = ((0, 0, 0), (1, 0, 0), (0, 0, 1), 0, 1)
+    for bad in ([((0, 0, 0), (1, 0, 0), (2, 0, 0), 0, 1)],          # parallel edges
+                [((0, 0, 0), (float("nan"), 0, 0), (0, 0, 1), 0, 1)],
+                [((0, 0, 0), (1, 0, 0), (0, 0, 1), 1 << 25, 1)]):
+        with pytest.raises(pbf_amd.PbfError) as e:
+            sim.set_collision_quads(bad)
+        assert e.value.code == pbf_amd._lib.PBF_E_INVALID
+    with pytest.raises(ValueError):
+        sim.set_collision_quads([ok] * 27)
+    with pytest.raises(pbf_amd.PbfError):
+        sim.set_collision_quads([ok], np.zeros((0, 4), np.uint32))
+    sim.set_collision_quads([ok] * 26)
+    sim.close()
+
+
 def test_deterministic_repeat():
     scene = pbf_amd.config("dam16k", seed=4)
     outs = []

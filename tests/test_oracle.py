@@ -318,6 +318,90 @@ def test_walls_single_particle(oracle):
     assert xs[0, 1] + delta[0, 1] == np.float32(-0.02)
 
 
+def _quad_case(xs_pos, prev_pos, mask, mask_y=2, mask_h=4, walls=False, edge_base=0.0):
+    """One particle, no friends (delta-p = 0), one floor quad at y = 0.2
+    over x, z in [0, 1]: E0 = (0, 0, 1) (s = z), E1 = (1, 0, 0) (t = x), so
+    n = E0 x E1 = (0, 1, 0).  Returns the final position and the hit bits."""
+    h = 0.1
+    x = np.zeros((1, 4), np.float32)
+    x[0, :3] = prev_pos
+    xs = np.zeros((1, 4), np.float32)
+    xs[0, :3] = xs_pos
+    p = pbf_amd.physical_params(1, h, (-1, -1, -1), (2, 2, 2))
+    c = po.cfg_from_params(p, pbf_amd.default_options(walls=walls))
+    po.set_quads(c, [((0.0, 0.2, 0.0), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0), mask_y, mask_h)], mask, edge_base)
+    fr = np.zeros(255, np.uint32)
+    delta = np.zeros_like(x)
+    hits = np.zeros(1, np.uint32)
+    po.lib().pbfo_delta_hits(ctypes.byref(c), po.ptr(x), po.ptr(xs), po.ptr(fr), po.ptr(delta), po.ptr(hits))
+    f = xs[0, :3] + delta[0, :3]
+    s = [po.lib().pbfo_sinf(float(np.float32(np.float32(v * np.float32(5.0)) / np.float32(h)))) for v in xs[0, :3]]
+    edge = np.float32(np.float32(np.float32(np.float32(np.float32(3.0 + s[0]) + s[1]) + s[2]) * np.float32(h)) * np.float32(0.03))
+    return f, int(hits[0]), np.float32(np.float32(edge_base) + edge)
+
+
+def test_quad_moves_a_point_onto_its_offset_plane(oracle):
+    """BouncePointQuad (compute_delta.cl:1-42) on an axis-aligned quad, where
+    s = z and t = x exactly: a point just through the quad, coming from
+    above, lands at y = 0.2 + edge with x and z unchanged; mask texel
+    ((int)(t W), mask_y + (int)(s mask_h)) = (2, 2 + 2) for t = 0.3, s = 0.7
+    in an 8-wide mask."""
+    mask = np.zeros((8, 8), np.uint32)
+    mask[4, 2] = 7
+    f, hit, eq = _quad_case((0.3, 0.199, 0.7), (0.3, 0.25, 0.7), mask, edge_base=0.25)
+    assert hit == 1 << 6
+    assert f[1] == np.float32(np.float32(0.199) + np.float32(np.float32(np.float32(0.2) + eq) - np.float32(0.199)))
+    assert abs(float(f[1]) - (0.2 + float(eq))) < 1e-7
+    assert f[0] == np.float32(0.3) and f[2] == np.float32(0.7)
+    # the neighbouring texel set instead: not solid there
+    mask[4, 2], mask[4, 3] = 0, 1
+    f, hit, _ = _quad_case((0.3, 0.199, 0.7), (0.3, 0.25, 0.7), mask)
+    assert hit == 0 and f[1] == np.float32(0.199)
+    # s = 0.7 -> row 2 + (int)(0.7 * 4) = 4; with mask_y = 3 the row is 5
+    mask[:] = 0
+    mask[5, 2] = 1
+    f, hit, _ = _quad_case((0.3, 0.199, 0.7), (0.3, 0.25, 0.7), mask, mask_y=3)
+    assert hit == 1 << 6
+
+
+def test_quad_rejections_and_mask_clamp(oracle):
+    full = np.ones((8, 8), np.uint32)
+    # outside the parallelogram (t = x = 1.2 > 1)
+    f, hit, _ = _quad_case((1.2, 0.199, 0.7), (1.2, 0.25, 0.7), full)
+    assert hit == 0 and f[1] == np.float32(0.199)
+    # in front of the offset plane (above 0.2 + edge)
+    f, hit, eq = _quad_case((0.3, 0.3, 0.7), (0.3, 0.35, 0.7), full)
+    assert 0.3 > 0.2 + eq and hit == 0
+    # deep below, barely moving: outside the force zone |dp| <= edge + n.(prev - F)
+    f, hit, _ = _quad_case((0.3, -0.5, 0.7), (0.3, -0.499, 0.7), full)
+    assert hit == 0 and f[1] == np.float32(-0.5)
+    # the same depth reached from above in one step: inside the zone
+    f, hit, _ = _quad_case((0.3, -0.5, 0.7), (0.3, 0.5, 0.7), full)
+    assert hit == 1 << 6
+    # t = 1 exactly gives column W = 8, clamped to the last column; s = 1
+    # gives row mask_y + mask_h = 6, inside the image
+    only = np.zeros((8, 8), np.uint32)
+    only[6, 7] = 1
+    f, hit, _ = _quad_case((1.0, 0.199, 1.0), (1.0, 0.25, 1.0), only)
+    assert hit == 1 << 6
+    # rows beyond the image clamp to the last row
+    only[:] = 0
+    only[7, 0] = 1
+    f, hit, _ = _quad_case((0.0, 0.199, 0.9), (0.0, 0.25, 0.9), only, mask_y=100)
+    assert hit == 1 << 6
+
+
+def test_quads_follow_the_walls(oracle):
+    """Walls first, then the quads with the walls' noise edge plus
+    edge_base (compute_delta.cl:166,169-182): a point through the box
+    floor at y = -1 and the quad is lifted by the floor, then left alone by
+    the quad far above it."""
+    full = np.ones((1, 1), np.uint32)
+    f, hit, _ = _quad_case((0.3, -1.01, 0.7), (0.3, -0.99, 0.7), full, walls=True)
+    assert hit == 1 << 2   # y-low plane only
+    assert f[1] > -1.0
+
+
 def test_step_thread_count_invariance(oracle, monkeypatch):
     """Bitwise identical results whatever the OpenMP thread count."""
     import subprocess
@@ -462,3 +546,31 @@ def test_vorticity_two_particle_closed_form(oracle):
     v[:] = 0.0
     po.lib().pbfo_vorticity(ctypes.byref(c), po.ptr(xs), po.ptr(v), po.ptr(om), po.ptr(fr))
     assert v[0, :3].tolist() == [0.0, 0.0, 0.0]
+
+
+def test_coincident_pair_has_no_direction(oracle):
+    """Two particles at one point: the pair counts in the density,
+    (h^2)^3 POLY6, but r/|r| is taken as 0, so its gradient terms vanish:
+    lambda = -C/eps and delta-p = 0 (the reference's 0/0 would give NaN)."""
+    h = 0.1
+    xs = np.zeros((2, 4), np.float32)
+    xs[:, :3] = (0.5, 0.5, 0.5)
+    fr = np.zeros(2 * 255, np.uint32)
+    fr[0:2] = 1
+    fr[5 * 2 + 0] = 1
+    fr[5 * 2 + 1] = 0
+    p = pbf_amd.physical_params(2, h, (0, 0, 0), (1, 1, 1))
+    c = po.cfg_from_params(p, pbf_amd.default_options(walls=False))
+    rho = np.zeros(2, np.float32)
+    lam = np.zeros(2, np.float32)
+    po.lib().pbfo_scaling(ctypes.byref(c), po.ptr(xs), po.ptr(fr), po.ptr(rho), po.ptr(lam))
+    poly6, _ = oracle.kernel_factors(h)
+    rho_want = poly6 * (h * h) ** 3
+    assert np.isfinite(lam).all()
+    assert abs(rho[0] / rho_want - 1) < 1e-6
+    lam_want = -(rho_want / float(p.restDensity) - 1) / float(p.epsilon)
+    assert abs(lam[0] / lam_want - 1) < 1e-6
+    xs[:, 3] = lam
+    delta = np.full((2, 4), 7.0, np.float32)
+    po.lib().pbfo_delta(ctypes.byref(c), po.ptr(xs), po.ptr(xs), po.ptr(fr), po.ptr(delta))
+    assert delta[:, :3].tolist() == [[0.0, 0.0, 0.0], [0.0, 0.0, 0.0]]

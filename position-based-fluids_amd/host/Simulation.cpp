@@ -68,6 +68,46 @@ void Simulation::CreateParticles() {
     Check(pbf_set_state(mCtx, x.data(), nullptr, n), "CreateParticles");
 }
 
+void Simulation::LoadForceMasks(const std::vector<uint32_t>& texels, uint32_t width, uint32_t height,
+                                const std::vector<pbf_quad>& quads, float edge_base) {
+    if (!mCtx) throw std::runtime_error("LoadForceMasks before InitBuffers");
+    if ((size_t)width * height != texels.size()) throw std::runtime_error("LoadForceMasks: texels != width x height");
+    Check(pbf_set_collision_quads(mCtx, 0, quads.data(), (uint32_t)quads.size(), texels.data(), width, height,
+                                  edge_base),
+          "LoadForceMasks");
+}
+
+std::vector<pbf_quad> Simulation::ForcePlates() {
+    // corner, E0, E1, mask rows (first, count) of each plate, in call order
+    static const float t[14][11] = {
+        {-165.8f, -106.4f, 28.32f, -15.39f, 41.82f, 0.0f, 52.28f, 19.23f, 0.0f, 0, 409},
+        {-158.5f, -126.3f, -37.11f, -7.638f, 20.76f, 66.1f, 52.05f, 19.15f, 0.0f, 409, 643},
+        {-21.79f, -23.62f, -34.55f, 89.42f, 39.44f, 0.0f, -27.26f, 61.81f, 0.0f, 1052, 740},
+        {-39.96f, -11.55f, 29.95f, -8.877f, 49.84f, 0.0f, 103.3f, 18.4f, 0.0f, 1792, 247},
+        {-180.5f, -64.4f, -37.25f, -0.0009671f, -0.0003558f, 66.38f, 22.87f, -62.17f, 0.0f, 2039, 512},
+        {35.5f, 2.278f, -35.25f, 0.0f, 0.0f, 65.91f, 21.74f, 36.07f, 0.0f, 2551, 801},
+        {-31.54f, 1.768f, -8.499f, 0.0f, 0.0f, 15.86f, -15.69f, -32.17f, 0.0f, 3352, 226},
+        {-67.22f, -41.65f, 7.364f, 11.45f, 0.0f, 0.0f, 0.0f, 0.0f, -15.86f, 3578, 369},
+        {-65.91f, -42.02f, 7.364f, 0.0f, 0.0f, -15.87f, -7.275f, 4.603f, 0.0f, 3947, 943},
+        {-9.05f, -4.783f, -8.792f, -47.98f, -37.3f, 0.0f, 0.0f, 0.0f, 16.45f, 4890, 1891},
+        {-53.98f, -25.37f, 6.656f, 27.86f, 34.59f, 0.0f, 17.42f, -14.03f, 0.0f, 6781, 1016},
+        {-48.37f, -30.96f, -7.792f, 21.75f, -10.61f, 0.0f, 17.92f, 36.76f, 0.0f, 7797, 303},
+        {-48.28f, 38.39f, -35.25f, 0.0f, 0.0f, 65.91f, 20.93f, -47.46f, 0.0f, 8100, 650},
+        {36.56f, 3.007f, 30.66f, 0.0f, 0.0f, -65.91f, -64.89f, -11.56f, 0.0f, 8750, 512},
+    };
+    std::vector<pbf_quad> out(14);
+    for (int k = 0; k < 14; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            out[k].origin[a] = t[k][a];
+            out[k].e0[a] = t[k][3 + a];
+            out[k].e1[a] = t[k][6 + a];
+        }
+        out[k].mask_y = (int32_t)t[k][9];
+        out[k].mask_h = (int32_t)t[k][10];
+    }
+    return out;
+}
+
 void Simulation::SetState(const std::vector<float>& x4, const std::vector<float>& v4) {
     if (!mCtx) throw std::runtime_error("SetState before InitBuffers");
     const uint32_t n = (uint32_t)(x4.size() / 4);

@@ -58,6 +58,16 @@ public:
     // src/Simulation.cpp:554-621
     void Step();
 
+    // src/Simulation.cpp:257-281: the force-plate mask (the red channel of
+    // Scene_fp_mask.png as width x height texels, row-major; the caller
+    // decodes the image, as SOIL did there) and the quads that sample it
+    // (by default the reference scene's plates, compute_delta.cl:169-182,
+    // with its edge offset 1.0).  Call after InitBuffers.
+    void LoadForceMasks(const std::vector<uint32_t>& texels, uint32_t width, uint32_t height,
+                        const std::vector<pbf_quad>& quads = ForcePlates(), float edge_base = 1.0f);
+    // The reference scene's 14 force plates (compute_delta.cl:169-182).
+    static std::vector<pbf_quad> ForcePlates();
+
     // Replace the particle state (float4 x and v, particle-major).
     void SetState(const std::vector<float>& x4, const std::vector<float>& v4);
     // Host staging of the two render inputs: the positions VBO (w = speed)

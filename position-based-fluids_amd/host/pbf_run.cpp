@@ -2,8 +2,11 @@
 // Runner::run sequence (src/Runner.cpp:43-132) without the renderer:
 // LoadParameters -> InitBuffers -> InitCells -> InitKernels -> subSteps x
 // Step per frame.  Prints one line of state statistics per reported frame.
+// With "ramp", LoadForceMasks installs one collision quad: a ramp over the
+// floor of the box (y = 0.4h at xMin up to 0.1 of the height at xMax),
+// solid everywhere.
 //
-//   pbf_run <scenario.par> <frames> [report_every]
+//   pbf_run <scenario.par> <frames> [report_every] [ramp]
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -15,7 +18,7 @@
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s <scenario.par> <frames> [report_every]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <scenario.par> <frames> [report_every] [ramp]\n", argv[0]);
         return 2;
     }
     std::ifstream f(argv[1]);
@@ -33,6 +36,18 @@ int main(int argc, char** argv) {
         sim.InitBuffers();
         sim.InitCells();
         if (!sim.InitKernels()) return 1;
+        if (argc > 4 && std::string(argv[4]) == "ramp") {
+            const pbf_params& P = pbf_host::Params;
+            pbf_quad q{};
+            q.origin[0] = P.xMin;
+            q.origin[1] = P.yMin + 0.4f * P.h;
+            q.origin[2] = P.zMin;
+            q.e0[2] = P.zMax - P.zMin;
+            q.e1[0] = P.xMax - P.xMin;
+            q.e1[1] = 0.1f * (P.yMax - P.yMin);
+            q.mask_h = 1;
+            sim.LoadForceMasks({1u}, 1, 1, {q}, 0.0f);
+        }
         std::vector<float> x, v;
         for (int frame = 1; frame <= frames; ++frame) {
             for (uint32_t s = 0; s < pbf_host::Params.subSteps; ++s) sim.Step();

@@ -195,6 +195,12 @@ template <int BS>
 struct FriendWriter {
     uint32_t packed = 0, dropped = 0;
     uint32_t* stage = nullptr;
+    uint4* qbase0 = nullptr;   // circle 0: the particle's sweep quads, quad q at q * 64
+    uint4* qbase1 = nullptr;   // circles 1..4: its circle quads, quad q of circle c at ((c - 1) * 13 + q) * 64
+    __device__ __forceinline__ void bind(uint32_t i, uint4* __restrict__ list, uint4* __restrict__ sweep) {
+        qbase0 = sweep + sweep_quad(i, 0);
+        qbase1 = list + friend_quad(i, 1, 0);
+    }
     __device__ __forceinline__ static uint4* quad_of(uint32_t i, int circ, int q, uint4* __restrict__ list,
                                                       uint4* __restrict__ sweep) {
         return circ == 0 ? sweep + sweep_quad(i, q) : list + friend_quad(i, circ, q);
@@ -209,7 +215,10 @@ struct FriendWriter {
         }
         uint32_t* col = stage + 4 * circ * BS;
         col[(f & 3u) * BS] = j;
-        if ((f & 3u) == 3u) *quad_of(i, circ, (int)(f >> 2), list, sweep) = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
+        if ((f & 3u) == 3u) {
+            const uint32_t qo = (circ == 0 ? (f >> 2) : (uint32_t)(circ - 1) * kSegQuads + (f >> 2)) * 64u;
+            (circ == 0 ? qbase0 : qbase1)[qo] = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
+        }
         packed += 1u << (6 * circ);
     }
     __device__ __forceinline__ void finish(uint32_t i, uint32_t* __restrict__ nb_cnt, uint4* __restrict__ list,
@@ -346,6 +355,7 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
     const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
     FriendWriter<BS> w;
     w.stage = s_stage + threadIdx.x;
+    w.bind(i, list, quads);
     const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
@@ -465,6 +475,7 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
     uint4* list = reinterpret_cast<uint4*>(nb_list);
     FriendWriter<BS> w;
     w.stage = s_stage + tid;
+    w.bind(i, list, quads);
     const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)

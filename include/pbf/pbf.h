@@ -194,6 +194,31 @@ int pbf_stats(pbf_ctx *ctx, pbf_stats_t *out);
  * clears them.  Synchronises. */
 int pbf_error_flags(pbf_ctx *ctx, uint32_t *flags, uint64_t *dropped_friends, int reset);
 
+/* Device event log: the role of the reference's device logger
+ * (assets/kernels/logging.cl, src/OCL_Logger.cpp:80-116), a ring in device
+ * memory that kernels append to and the host reads after a step.  Each
+ * message is {code, a, b, c}; the kernels log: */
+#define PBF_LOG_FRIENDS_OVERFLOW 1u /* a = particle (current order), b = friends
+                                       dropped, c = its packed circle counts
+                                       (6 bits per circle)                      */
+#define PBF_LOG_NAN 2u              /* a = first particle of a workgroup with a
+                                       non-finite predicted position, b = how
+                                       many in that workgroup, c = scene        */
+#define PBF_LOG_SORT_TIMEOUT 3u     /* a = tile, b = digit, c = key shift        */
+#define PBF_LOG_INTERNAL 4u         /* a = where (1 gather, 2 cells, 3 sort
+                                       scatter), b = index, c = bad value       */
+#define PBF_LOG_CAPACITY 1024u      /* messages the ring holds                   */
+
+typedef struct pbf_log_msg {
+    uint32_t code, a, b, c;
+} pbf_log_msg;
+
+/* Messages logged since the previous call, oldest first: up to cap into out
+ * (*count receives how many; the rest stay for the next call).  *lost (may
+ * be NULL) receives how many were overwritten before they were read (more
+ * than PBF_LOG_CAPACITY between reads).  Synchronises. */
+int pbf_read_log(pbf_ctx *ctx, pbf_log_msg *out, uint32_t cap, uint32_t *count, uint64_t *lost);
+
 /* The r^2 at which the friend circle of build_friends_list.cl:75-76 reaches
  * 1, 2, 3, 4 for smoothing length h (+inf if never in range): the kernels
  * classify a friend by comparing its binary32 r^2 against these. */

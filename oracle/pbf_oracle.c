@@ -223,7 +223,9 @@ void pbfo_friends(const pbfo_cfg *c, const float *xs, const uint32_t *cells, uin
                         const float *pj = F4(xs, j);
                         float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
                         const float r2 = dot3(r, r);
-                        if (r2 >= h2) continue;
+                        /* build_friends_list.cl:70 skips r^2 >= h^2; a NaN
+                         * r^2 (relaxed math there: undefined) is skipped too */
+                        if (!(r2 < h2)) continue;
                         const float a = fmaxf(0.0f, (sqrtf(r2) - min_r) / span);
                         int circ = (int)(a * a * a * (float)PBFO_CIRCLES);
                         if (circ > PBFO_CIRCLES - 1) circ = PBFO_CIRCLES - 1;

@@ -252,7 +252,7 @@ int create_impl(const pbf_params* params, const pbf_options* opts, uint32_t S, p
     CREATE_TRY(dalloc(&ctx->nb_cnt, n));
     CREATE_TRY(dalloc(&ctx->nb_list, pbf::friend_slots(n)));
     CREATE_TRY(dalloc(&ctx->nb_quads, ((n + 63) / 64) * 64 * (size_t)pbf::kQuads));
-    CREATE_TRY(dalloc(&ctx->err, pbf::kErrWords));
+    CREATE_TRY(dalloc(&ctx->err, pbf::kErrAllocWords));
     CREATE_TRY(dalloc(&ctx->sc_d, S));
     CREATE_TRY(hipMemcpy(ctx->sc_d, ctx->sc.data(), sizeof(pbf::SceneConsts) * S, hipMemcpyHostToDevice));
 
@@ -289,7 +289,7 @@ int create_impl(const pbf_params* params, const pbf_options* opts, uint32_t S, p
     CREATE_TRY(hipMemsetAsync(ctx->cells, 0xFF, 8 * (size_t)ctx->total_cells, ctx->stream));
     CREATE_TRY(hipMemsetAsync(ctx->nb_cnt, 0, 4 * std::max<size_t>(n, 1), ctx->stream));
     CREATE_TRY(hipMemsetAsync(ctx->v, 0, 16 * std::max<size_t>(n, 1), ctx->stream));
-    CREATE_TRY(hipMemsetAsync(ctx->err, 0, 4 * pbf::kErrWords, ctx->stream));
+    CREATE_TRY(hipMemsetAsync(ctx->err, 0, 4 * pbf::kErrAllocWords, ctx->stream));
     CREATE_TRY(hipStreamSynchronize(ctx->stream));
 #undef CREATE_TRY
     *out = ctx;
@@ -737,6 +737,32 @@ int pbf_error_flags(pbf_ctx* ctx, uint32_t* flags, uint64_t* dropped, int reset)
         HIP_TRY(ctx, hipMemsetAsync(ctx->err, 0, sizeof w, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
+    return PBF_OK;
+}
+
+int pbf_read_log(pbf_ctx* ctx, pbf_log_msg* out, uint32_t cap, uint32_t* count, uint64_t* lost) {
+    if (!ctx || !count || (cap && !out)) return PBF_E_INVALID;
+    static_assert(pbf::kLogRing == 4 * PBF_LOG_CAPACITY, "four words per message");
+    std::vector<uint32_t> w(pbf::kLogRing);
+    uint32_t at = 0;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(&at, ctx->err + pbf::kLogIndex, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(w.data(), ctx->err + pbf::kLogRingStart, 4 * w.size(), hipMemcpyDeviceToHost));
+    // indices are in words and advance by 4; unsigned differences survive the wrap
+    uint32_t from = ctx->log_read;
+    uint64_t missed = 0;
+    if (at - from > pbf::kLogRing) {
+        missed = (at - from - pbf::kLogRing) / 4;
+        from = at - pbf::kLogRing;
+    }
+    uint32_t k = 0;
+    for (; k < cap && from != at; ++k, from += 4) {
+        const uint32_t* m = w.data() + (from % pbf::kLogRing);
+        out[k] = {m[0], m[1], m[2], m[3]};
+    }
+    ctx->log_read = from;
+    *count = k;
+    if (lost) *lost = missed;
     return PBF_OK;
 }
 

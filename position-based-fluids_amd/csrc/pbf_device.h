@@ -14,6 +14,22 @@ constexpr int kCircles = 5;          // Params.friendsCircles (src/ParamUtils.cp
 constexpr int kPerCircle = 50;       // Params.particlesPerCircle (src/ParamUtils.cpp:99)
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // END_OF_CELL_LIST (src/Simulation.cpp:170)
 
+// Event log (the reference's device logger, logging.cl: a ring in device
+// memory with an atomically advanced write index, read by the host after a
+// step).  In the error-word allocation: word kLogIndex is the write index
+// (monotonic, in words); the ring of kLogRing words starts at kLogRingStart.
+// Every message is four words {code, a, b, c} (PBF_LOG_*), so messages never
+// straddle the ring's end and the index stays aligned across its wrap.
+constexpr uint32_t kLogIndex = 16;
+constexpr uint32_t kLogRingStart = 20;
+constexpr uint32_t kLogRing = 4096;
+constexpr uint32_t kErrAllocWords = kLogRingStart + kLogRing;
+
+__device__ __forceinline__ void log_event(uint32_t* err, uint32_t code, uint32_t a, uint32_t b, uint32_t c) {
+    const uint32_t at = atomicAdd(err + kLogIndex, 4u);
+    *reinterpret_cast<uint4*>(err + kLogRingStart + (at % kLogRing)) = make_uint4(code, a, b, c);
+}
+
 constexpr uint32_t kMaxQuads = 26;   // hit bits 6..31 of a particle's boundary record
 
 // One collision quad with the constants BouncePointQuad derives from it

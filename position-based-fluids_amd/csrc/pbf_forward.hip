@@ -56,11 +56,15 @@ __global__ __launch_bounds__(kPredictThreads) void k_predict_hash(
     uint2* __restrict__ cells, uint32_t reset_old, uint32_t total_cells, uint32_t* __restrict__ hist,
     uint32_t passes, uint32_t pause, uint32_t* err) {
     __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+    __shared__ uint32_t s_bad[2];   // first non-finite particle of the workgroup, their count
     const uint4 be = blocks[blockIdx.x];
     const SceneConsts& c = scenes[be.x];
     for (int q = threadIdx.x; q < (int)passes * kRadix; q += kPredictThreads) s_hist[q] = 0;
+    if (threadIdx.x == 0) {
+        s_bad[0] = 0xFFFFFFFFu;
+        s_bad[1] = 0u;
+    }
     __syncthreads();
-    bool bad = false;
 #pragma unroll
     for (int k = 0; k < kPredictItems; ++k) {
         const uint32_t i = be.y + (uint32_t)(k * kPredictThreads) + threadIdx.x;
@@ -75,7 +79,10 @@ __global__ __launch_bounds__(kPredictThreads) void k_predict_hash(
                 const uint32_t ok = old_keys[i];
                 if ((i == 0 || old_keys[i - 1] != ok) && ok < total_cells) cells[ok] = make_uint2(kEmpty, kEmpty);
             }
-            bad |= !(fabsf(p.x) <= 3.4e38f && fabsf(p.y) <= 3.4e38f && fabsf(p.z) <= 3.4e38f);
+            if (!(fabsf(p.x) <= 3.4e38f && fabsf(p.y) <= 3.4e38f && fabsf(p.z) <= 3.4e38f)) {
+                atomicMin(&s_bad[0], i);
+                atomicAdd(&s_bad[1], 1u);
+            }
         }
     }
     __syncthreads();
@@ -83,7 +90,10 @@ __global__ __launch_bounds__(kPredictThreads) void k_predict_hash(
         const uint32_t h = s_hist[q];
         if (h) atomicAdd(&hist[q], h);
     }
-    if (bad) atomicOr(err, (uint32_t)PBF_FLAG_NAN);
+    if (threadIdx.x == 0 && s_bad[1]) {
+        atomicOr(err, (uint32_t)PBF_FLAG_NAN);
+        log_event(err, PBF_LOG_NAN, s_bad[0], s_bad[1], be.x);
+    }
 }
 
 // Paused step with a valid sort: predictPositions only (no re-sort).
@@ -114,6 +124,7 @@ __global__ __launch_bounds__(256) void k_gather(const SceneConsts* __restrict__ 
     uint32_t p = perm[i];
     if (p >= n) {  // only after a failed sort (flagged there)
         atomicOr(err, (uint32_t)PBF_FLAG_INTERNAL);
+        log_event(err, PBF_LOG_INTERNAL, 1u, i, p);
         p = i;
     }
     const float4 xp = x[p];
@@ -130,6 +141,7 @@ __global__ __launch_bounds__(256) void k_cells(const uint32_t* __restrict__ ks, 
     const uint32_t k = ks[i];
     if (k >= total_cells || (i > 0 && ks[i - 1] >= total_cells)) {  // only after a failed sort
         atomicOr(err, (uint32_t)PBF_FLAG_INTERNAL);
+        log_event(err, PBF_LOG_INTERNAL, 2u, i, k);
         return;
     }
     if (i == 0) {
@@ -173,7 +185,10 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// Circle of a candidate in range, or -1 when r^2 >= h^2.  The reference's
+// Circle of a candidate in range, or -1 when r^2 >= h^2 or r^2 is NaN (a
+// friend needs r^2 < h^2; the reference's relaxed-math build leaves NaN
+// comparisons undefined, and skipping NaN keeps the cell culling exact).  The
+// reference's
 // circle, min(int(a^3 * 5), 4) with a = max(0, (sqrt(r^2) - 0.3h) / 0.7h)
 // (build_friends_list.cl:75-76), is a non-decreasing step function of the
 // binary32 r^2 (every operation in it is monotone), so it equals the number
@@ -182,7 +197,7 @@ __device__ __forceinline__ int wave_max(int v) {
 __device__ __forceinline__ int friend_circle(const float4& p, const float4& q, const SceneConsts& c) {
     const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 >= c.h2) return -1;
+    if (!(r2 < c.h2)) return -1;
     return (int)(r2 >= c.circ_t[0]) + (int)(r2 >= c.circ_t[1]) + (int)(r2 >= c.circ_t[2]) + (int)(r2 >= c.circ_t[3]);
 }
 
@@ -235,6 +250,7 @@ struct FriendWriter {
         if (dropped) {
             atomicOr(err, (uint32_t)PBF_FLAG_FRIENDS_OVERFLOW);
             atomicAdd((unsigned long long*)(err + 2), (unsigned long long)dropped);
+            log_event(err, PBF_LOG_FRIENDS_OVERFLOW, i, dropped, packed);
         }
     }
 };
@@ -519,10 +535,10 @@ struct Pair4 {
 // thread of this kernel reads another particle's w).
 struct LambdaAcc {
     float dens = 0.0f, gsum = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    // Branch-free: an out-of-range pair has its scalar factors set to 0, so it
-    // adds +0 or -0 (no sum here is ever -0, so the result is bitwise that of
-    // skipping it) and no branch joins the loop, which keeps the next gathers
-    // in flight.
+    // Branch-free: an out-of-range pair's contributions are selected to +0
+    // (no sum here is ever -0, so the result is bitwise that of skipping it,
+    // also when the friend's position is not finite) and no branch joins the
+    // loop, which keeps the next gathers in flight.
     __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -531,8 +547,8 @@ struct LambdaAcc {
         const float hr = c.h - rl;
         const float s = c.spiky * hr * hr;
         const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   // r/|r| = r * (1/|r|); 0 at r = 0
-        const float t = in ? s * inv : 0.0f;
-        const float ax = rx * t, ay = ry * t, az = rz * t;
+        const float t = s * inv;
+        const float ax = in ? rx * t : 0.0f, ay = in ? ry * t : 0.0f, az = in ? rz * t : 0.0f;
         const float d = in ? c.h2 - r2 : 0.0f;
         dens += d * d * d;
         gsum += ax * ax + ay * ay + az * az;
@@ -585,10 +601,10 @@ struct DeltaAcc {
         const float p6 = d * d * d;
         const float ratio = p6 * c.inv_poly6_q;
         const float scorr = c.st_k * ratio * ratio * ratio * ratio;
-        const float coef = in ? p.w + pj.w + scorr : 0.0f;
-        sx += coef * ax;
-        sy += coef * ay;
-        sz += coef * az;
+        const float coef = p.w + pj.w + scorr;
+        sx += in ? coef * ax : 0.0f;
+        sy += in ? coef * ay : 0.0f;
+        sz += in ? coef * az : 0.0f;
     }
 };
 
@@ -643,23 +659,22 @@ struct ViscAcc {
                                         const SceneConsts& c) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        // masked through the geometric factors, which are finite whenever
-        // the pair is skipped (a skipped pair at r = 0 would give 0 * inf)
+        // a skipped pair's contributions are selected to +0 (as in LambdaAcc)
         const bool in = r2 < c.h2 && fabsf(pj.w) > 1e-8f;
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float d = c.h2 - r2;
         const float rl = sqrtf(r2);
         const float hr = c.h - rl;
         const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;
-        const float t = in ? inv * hr * hr : 0.0f;
+        const float t = inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
-        ox += uy * az - uz * ay;
-        oy += uz * ax - ux * az;
-        oz += ux * ay - uy * ax;
-        const float p6 = in ? d * d * d : 0.0f;
-        vsx += ux * p6;
-        vsy += uy * p6;
-        vsz += uz * p6;
+        ox += in ? uy * az - uz * ay : 0.0f;
+        oy += in ? uz * ax - ux * az : 0.0f;
+        oz += in ? ux * ay - uy * ax : 0.0f;
+        const float p6 = d * d * d;
+        vsx += in ? ux * p6 : 0.0f;
+        vsy += in ? uy * p6 : 0.0f;
+        vsz += in ? uz * p6 : 0.0f;
     }
 };
 
@@ -698,12 +713,12 @@ struct VortAcc {
         const float rl = sqrtf(r2);
         const float hr = c.h - rl;
         const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;
-        const float t = in ? inv * hr * hr : 0.0f;
+        const float t = inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
         const float wl = sqrtf(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
-        ex += wl * ax;
-        ey += wl * ay;
-        ez += wl * az;
+        ex += in ? wl * ax : 0.0f;
+        ey += in ? wl * ay : 0.0f;
+        ez += in ? wl * az : 0.0f;
     }
 };
 

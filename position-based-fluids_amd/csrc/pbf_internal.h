@@ -21,7 +21,9 @@ constexpr int kRadix = 1 << kRadixBits;
 constexpr int kMaxPasses = 4;
 
 // Device error words: [0] PBF_FLAG_* bits, [2..3] dropped-friend count (u64),
-// [4] tiles whose neighbourhood did not fit in LDS (diagnostic).
+// [4] tiles whose neighbourhood did not fit in LDS (diagnostic).  The same
+// allocation holds the event log after them (kLogIndex, kLogRing in
+// pbf_device.h).
 constexpr int kErrWords = 8;
 
 // Per-particle kernels run over a block table: one uint4 {scene, first,
@@ -120,7 +122,8 @@ struct pbf_ctx {
     uint32_t* nb_list = nullptr;     // friends, [n/64][5 circles][kSeg][64]
     uint4* nb_quads = nullptr;       // processed friends, sweep quads
 
-    uint32_t* err = nullptr;
+    uint32_t* err = nullptr;     // error words, then the event log (kErrAllocWords)
+    uint32_t log_read = 0;       // log write index up to which the host has read
 
     // collision quads (pbf_set_collision_quads), per scene on the host; one
     // device buffer holds every scene's QuadConsts and mask bits, and the

@@ -121,6 +121,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint32_t* __res
             if (flag == 0u) {
                 if (++spins > kSpinLimit) {
                     atomicOr(err, (uint32_t)PBF_FLAG_SORT_TIMEOUT);
+                    log_event(err, PBF_LOG_SORT_TIMEOUT, tile, d, shift);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
@@ -146,6 +147,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint32_t* __res
                 vout[dst] = val[k];
             } else {
                 atomicOr(err, (uint32_t)PBF_FLAG_INTERNAL);
+                log_event(err, PBF_LOG_INTERNAL, 3u, idx, dst);
             }
         }
     }

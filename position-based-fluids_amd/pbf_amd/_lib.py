@@ -92,6 +92,18 @@ class PbfQuad(ctypes.Structure):
 PBF_MAX_QUADS = 26
 
 
+class PbfLogMsg(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_uint32), ("a", ctypes.c_uint32), ("b", ctypes.c_uint32), ("c", ctypes.c_uint32)]
+
+
+PBF_LOG_FRIENDS_OVERFLOW = 1
+PBF_LOG_NAN = 2
+PBF_LOG_SORT_TIMEOUT = 3
+PBF_LOG_INTERNAL = 4
+PBF_LOG_CAPACITY = 1024
+LOG_NAMES = {1: "friends_overflow", 2: "nan", 3: "sort_timeout", 4: "internal"}
+
+
 class PbfTiming(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 48),
@@ -166,6 +178,8 @@ _SIGNATURES = {
     "pbf_set_friends_mode": (ctypes.c_int, [_VP, ctypes.c_int]),
     "pbf_autotune": (ctypes.c_int, [_VP, ctypes.c_uint32, _UP]),
     "pbf_stats": (ctypes.c_int, [_VP, ctypes.POINTER(PbfStats)]),
+    "pbf_read_log": (ctypes.c_int, [_VP, ctypes.POINTER(PbfLogMsg), ctypes.c_uint32, _UP,
+                                    ctypes.POINTER(ctypes.c_uint64)]),
     "pbf_friend_circle_thresholds": (ctypes.c_int, [ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "pbf_copy_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "pbf_set_state": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32]),

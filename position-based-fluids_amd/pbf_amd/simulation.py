@@ -11,7 +11,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import FAMILIES, PbfDevBufs, PbfError, PbfOptions, PbfParams, PbfStats, PbfTiming, check, lib
+from ._lib import (FAMILIES, LOG_NAMES, PbfDevBufs, PbfError, PbfLogMsg, PbfOptions, PbfParams, PbfStats, PbfTiming,
+                   check, lib)
 from .scenes import default_options
 
 _CIRCLES = 5
@@ -187,6 +188,17 @@ class Simulation:
         d = ctypes.c_uint64()
         check(lib().pbf_error_flags(self._ctx, ctypes.byref(f), ctypes.byref(d), 1 if reset else 0), self._ctx)
         return int(f.value), int(d.value)
+
+    def read_log(self, cap: int = 1024) -> tuple[list[dict], int]:
+        """Device event log since the previous read (pbf_read_log): messages
+        as dicts {event, a, b, c}, oldest first, and how many were lost to
+        the ring's wrap."""
+        self._ensure()
+        arr = (PbfLogMsg * max(cap, 1))()
+        cnt, lost = ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().pbf_read_log(self._ctx, arr, cap, ctypes.byref(cnt), ctypes.byref(lost)), self._ctx)
+        return ([{"event": LOG_NAMES.get(m.code, str(m.code)), "code": int(m.code), "a": int(m.a), "b": int(m.b),
+                  "c": int(m.c)} for m in arr[:cnt.value]], int(lost.value))
 
     def close(self):
         if self._ctx is not None:

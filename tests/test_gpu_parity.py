@@ -150,6 +150,60 @@ def test_friend_overflow_flag():
     # every particle sees n-1 friends in range; all beyond 50 per circle drop
     assert dropped == int(n * (n - 1) - counts.sum())
     assert sim.error_flags() == (0, 0)
+    # the event log names every particle, with its drop count and circle counts
+    log, lost = sim.read_log()
+    assert lost == 0 and len(log) == n
+    assert sorted(m["a"] for m in log) == list(range(n))
+    assert all(m["event"] == "friends_overflow" and m["b"] == n - 1 - 50 and m["c"] == 50 for m in log)
+    assert sim.read_log() == ([], 0)
+    sim.close()
+
+
+def test_non_finite_particles_match_oracle():
+    # a NaN and an infinite particle land in friend lists (as in the reference,
+    # whose r^2 >= h^2 test lets NaN through); the sweeps skip their pairs like
+    # the oracle's branches, so the finite particles stay finite and bitwise
+    # equal to the oracle
+    scene = pbf_amd.config("dam4k", seed=3)
+    x = scene.x.copy()
+    x[7, 0] = np.nan
+    x[100, 2] = np.inf
+    scene = pbf_amd.Scene(scene.params, scene.options, x, np.zeros_like(x))
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 3)
+    check_structure(sim, ref)
+    a, b = check_state(sim, ref, scene)
+    assert np.isfinite(a["x"][:, :3]).all(axis=1).sum() >= scene.n - 2
+    sim.close()
+
+
+def test_event_log_nan_and_wrap():
+    # a non-finite position is logged once for its workgroup; the ring keeps the
+    # last 1024 messages and counts the overwritten ones
+    scene = pbf_amd.config("dam4k", seed=2)
+    x = scene.x.copy()
+    x[[5, 9], 1] = np.nan
+    sim = pbf_amd.Simulation(scene.params, scene.options)
+    sim.set_state(x, scene.v)
+    sim.Step()
+    log, lost = sim.read_log()
+    nan = [m for m in log if m["event"] == "nan"]
+    assert lost == 0 and len(nan) == 1 and nan[0]["a"] == 5 and nan[0]["b"] == 2 and nan[0]["c"] == 0
+    assert sim.error_flags()[0] & pbf_amd.PBF_FLAG_NAN
+    sim.close()
+    # 1,200 overflowing particles in one step: 176 messages are lost
+    rng = np.random.default_rng(1)
+    n = 1200
+    xo = np.zeros((n, 4), np.float32)
+    xo[:, :3] = (0.55 + 0.02 * rng.random((n, 3))).astype(np.float32)
+    p = pbf_amd.physical_params(n, 0.1, (0, 0, 0), (1.0, 1.0, 1.0), iters=1)
+    sim = pbf_amd.Simulation(p, pbf_amd.default_options())
+    sim.set_state(xo, np.zeros_like(xo))
+    sim.Step()
+    first, lost = sim.read_log(cap=1000)
+    rest, lost2 = sim.read_log()
+    assert lost == n - 1024 and lost2 == 0 and len(first) == 1000 and len(rest) == 24
+    assert len({m["a"] for m in first + rest}) == 1024
     sim.close()
 
 

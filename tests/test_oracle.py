@@ -548,6 +548,35 @@ def test_vorticity_two_particle_closed_form(oracle):
     assert v[0, :3].tolist() == [0.0, 0.0, 0.0]
 
 
+def test_nan_particle_is_nobody_s_friend(oracle):
+    """A friend needs r^2 < h^2: a particle with a NaN coordinate has no
+    friends and is in no list, while the finite particles keep theirs.  With
+    gridBufSize 64 the NaN particle's cell (0, 5, 5) and the others' cell
+    (5, 5, 5) hash into each other's neighbourhoods, so both pairs are
+    tested."""
+    h = 0.1
+    n = 3
+    xs = np.zeros((n, 4), np.float32)
+    xs[0, :3] = (0.5, 0.5, 0.5)
+    xs[1, :3] = (0.53, 0.5, 0.5)
+    xs[2, :3] = (np.nan, 0.5, 0.5)
+    p = pbf_amd.physical_params(n, h, (0, 0, 0), (1, 1, 1), grid_buf_size=64)
+    c = po.cfg_from_params(p, pbf_amd.default_options())
+    cells = np.full(2 * 64, 0xFFFFFFFF, np.uint32)
+    for i in range(n):   # one bucket per particle, as the hash gives them
+        k = po.lib().pbfo_cell_hash(*[int(po.lib().pbfo_cell_coord(float(np.float32(v) / np.float32(h))))
+                                      for v in xs[i, :3]], 64)
+        cells[2 * k] = min(cells[2 * k], i) if cells[2 * k] != 0xFFFFFFFF else i
+        cells[2 * k + 1] = i
+    fr = np.zeros(n * 255, np.uint32)
+    po.lib().pbfo_friends(ctypes.byref(c), po.ptr(xs), po.ptr(cells), po.ptr(fr))
+    counts = fr[: 5 * n].reshape(5, n)
+    idx = fr[5 * n:].reshape(5, 50, n)
+    assert counts[:, 2].sum() == 0
+    assert counts[:, 0].sum() == 1 and counts[:, 1].sum() == 1
+    assert idx[0, 0, 0] == 1 and idx[0, 0, 1] == 0
+
+
 def test_coincident_pair_has_no_direction(oracle):
     """Two particles at one point: the pair counts in the density,
     (h^2)^3 POLY6, but r/|r| is taken as 0, so its gradient terms vanish:

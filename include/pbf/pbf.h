@@ -80,6 +80,17 @@ int pbf_scene_offsets(pbf_ctx *ctx, uint32_t *offsets /* scenes + 1 */);
  * (zero velocities). */
 int pbf_set_state(pbf_ctx *ctx, const float *x4, const float *v4, uint32_t n);
 
+/* Live parameter change, the hot-reload path of Runner::run
+ * (src/Runner.cpp:43-77: a changed scenario is reloaded and the kernels are
+ * rebuilt with its constants, src/Simulation.cpp:129-204): scene `scene`
+ * takes params for its next step (h, timeStep, restDensity, epsilon,
+ * garvity, viscosity/vorticity factors, surface tension, box bounds) and the
+ * next step re-sorts even when paused.  particleCount, gridBufSize and
+ * simIterations size the context and must not change (PBF_E_INVALID: create
+ * a new context, as the reference re-runs InitBuffers).  The particle state,
+ * options and collision quads are kept.  Waits for queued work. */
+int pbf_update_params(pbf_ctx *ctx, uint32_t scene, const pbf_params *params);
+
 /* One collision quad: corner B, edges E0 and E1 (BouncePointQuad's
  * arguments, compute_delta.cl:1) and the band of mask rows it samples:
  * texel row mask_y + (int)(s * mask_h) at quad parameter s along E0. */

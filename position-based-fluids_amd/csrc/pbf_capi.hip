@@ -571,6 +571,32 @@ int pbf_set_state(pbf_ctx* ctx, const float* x4, const float* v4, uint32_t n) {
     return PBF_OK;
 }
 
+int pbf_update_params(pbf_ctx* ctx, uint32_t scene, const pbf_params* params) {
+    if (!ctx) return PBF_E_INVALID;
+    if (scene >= ctx->S) return fail(ctx, PBF_E_INVALID, "scene %u out of range (%u scenes)", scene, ctx->S);
+    const int rc = validate(params, ctx, scene);
+    if (rc) return rc;
+    const pbf_params& old = ctx->P[scene];
+    if (params->particleCount != old.particleCount || params->gridBufSize != old.gridBufSize ||
+        params->simIterations != old.simIterations)
+        return fail(ctx, PBF_E_INVALID,
+                    "scene %u: particleCount, gridBufSize and simIterations size the context; create a new one", scene);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const pbf::SceneConsts& prev = ctx->sc[scene];
+    pbf::SceneConsts c = make_consts(*params, ctx->opt[scene], prev.cell_base, prev.first);
+    c.quads = prev.quads;
+    c.mask = prev.mask;
+    c.quad_count = prev.quad_count;
+    c.mask_w = prev.mask_w;
+    c.mask_rows = prev.mask_rows;
+    c.edge_base = prev.edge_base;
+    HIP_TRY(ctx, hipMemcpy(ctx->sc_d + scene, &c, sizeof c, hipMemcpyHostToDevice));
+    ctx->sc[scene] = c;
+    ctx->P[scene] = *params;
+    ctx->order_sorted = false;   // h may have moved every particle's cell: sort before the next friends build
+    return PBF_OK;
+}
+
 int pbf_set_collision_quads(pbf_ctx* ctx, uint32_t scene, const pbf_quad* quads, uint32_t count, const uint32_t* mask,
                             uint32_t mask_w, uint32_t mask_rows, float edge_base) {
     if (!ctx) return PBF_E_INVALID;

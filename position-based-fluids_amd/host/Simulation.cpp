@@ -2,7 +2,13 @@
 // pbf C ABI (see Simulation.hpp).
 #include "Simulation.hpp"
 
+#include <fcntl.h>
+#include <sys/file.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 namespace pbf_host {
@@ -43,7 +49,60 @@ void Simulation::InitCells() {
     // pbf_create leaves every cell empty and every friends count at zero.
 }
 
-bool Simulation::InitKernels() { return mCtx != nullptr; }
+bool Simulation::InitKernels() {
+    if (!mCtx) return false;
+    pbf_params p = Params;
+    pbf_params_finalize(&p);
+    return pbf_update_params(mCtx, 0, &p) == PBF_OK;
+}
+
+void Simulation::ReadLog(std::vector<pbf_log_msg>* out, uint64_t* lost) {
+    if (!mCtx) throw std::runtime_error("ReadLog before InitBuffers");
+    out->clear();
+    uint64_t total_lost = 0;
+    for (;;) {
+        pbf_log_msg buf[256];
+        uint32_t count = 0;
+        uint64_t l = 0;
+        Check(pbf_read_log(mCtx, buf, 256, &count, &l), "ReadLog");
+        total_lost += l;
+        out->insert(out->end(), buf, buf + count);
+        if (count < 256) break;
+    }
+    if (lost) *lost = total_lost;
+}
+
+bool DetectResourceChanges(std::list<std::pair<std::string, time_t>>& fileList) {
+    bool changed = false;
+    for (auto& f : fileList) {
+        struct stat st;
+        if (stat(f.first.c_str(), &st) != 0) {
+            std::fprintf(stderr, "Unable to locate %s\n", f.first.c_str());
+            continue;
+        }
+        if (f.second != st.st_mtime) {
+            changed = true;
+            break;
+        }
+    }
+    if (!changed) return false;
+    // a writer may still hold a file: report the change once all can be locked
+    for (auto& f : fileList) {
+        const int fd = open(f.first.c_str(), O_RDWR);
+        const int locked = fd < 0 ? -1 : flock(fd, LOCK_EX | LOCK_NB);
+        if (locked == 0) flock(fd, LOCK_UN);
+        if (fd >= 0) close(fd);
+        if (locked != 0) {
+            std::fprintf(stderr, "File is exclusivly open [%s]\n", f.first.c_str());
+            return false;
+        }
+    }
+    for (auto& f : fileList) {
+        struct stat st;
+        if (stat(f.first.c_str(), &st) == 0) f.second = st.st_mtime;
+    }
+    return true;
+}
 
 // src/Simulation.cpp:71-101: a cube of ceil(N^(1/3)) particles per axis at
 // spacing h * setupSpacing, offset (1 - axis*d)/2 in x and z and 0.3 in y,

@@ -11,6 +11,8 @@
 #pragma once
 
 #include <cstdint>
+#include <ctime>
+#include <list>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -26,6 +28,13 @@ static const int END_OF_CELL_LIST = -1;
 // A global parameter object and loader (src/ParamUtils.hpp:10-13).
 extern pbf_params Params;
 void LoadParameters(const std::string& text);
+
+// src/Resources.cpp:166-238: true when a tracked file's modification time
+// differs from the stored one and every tracked file can be locked
+// exclusively (no writer still holds it); the stored times are then
+// updated.  A file tracked with time 0 reports a change on the first call,
+// which is how the reference's frame loop does its initial load.
+bool DetectResourceChanges(std::list<std::pair<std::string, time_t>>& fileList);
 
 // OCLPerfMon::Trackers equivalent (src/OCLPerfMon.h).
 struct PerfTracker {
@@ -52,8 +61,11 @@ public:
     void InitBuffers();
     // src/Simulation.cpp:246-255: cells to END_OF_CELL_LIST, friends to 0.
     void InitCells();
-    // src/Simulation.cpp:129-204: nothing to compile at run time (the HIP
-    // code objects are built ahead of time); returns true.
+    // src/Simulation.cpp:129-204: the reference compiles its kernels with
+    // the current Params baked in.  The HIP code objects are built ahead of
+    // time, so this applies Params to the live context instead
+    // (pbf_update_params); false when the context cannot take them (a changed
+    // particleCount, gridBufSize or simIterations needs InitBuffers first).
     bool InitKernels();
     // src/Simulation.cpp:554-621
     void Step();
@@ -74,6 +86,10 @@ public:
     // and the 2048-wide RGBA32F texture contents (x*, w = density).
     void CopyRenderBuffers(std::vector<float>* positions4, std::vector<float>* texture4) const;
     void GetState(std::vector<float>* x4, std::vector<float>* v4) const;
+    // Device event log since the last read (pbf_read_log), the role of
+    // OCL_Logger::CycleExecute (src/OCL_Logger.cpp:80-116); *lost counts the
+    // messages the ring overwrote before they were read.
+    void ReadLog(std::vector<pbf_log_msg>* out, uint64_t* lost = nullptr);
     // Collect per-stage timings into PerfData (OCLPerfMon::UpdateTimings).
     void UpdateTimings();
 

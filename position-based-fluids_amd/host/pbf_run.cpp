@@ -1,55 +1,117 @@
 // pbf_run -- headless frame loop over the C++ Simulation drop-in, the
-// Runner::run sequence (src/Runner.cpp:43-132) without the renderer:
-// LoadParameters -> InitBuffers -> InitCells -> InitKernels -> subSteps x
-// Step per frame.  Prints one line of state statistics per reported frame.
-// With "ramp", LoadForceMasks installs one collision quad: a ramp over the
-// floor of the box (y = 0.4h at xMin up to 0.1 of the height at xMax),
-// solid everywhere.
+// Runner::run sequence (src/Runner.cpp:43-132) without the renderer: when
+// the scenario file changes (DetectResourceChanges; always on the first
+// frame) LoadParameters, then InitBuffers if the particle count changed or
+// ResetSimOnChange is set, InitCells, InitKernels; then subSteps x Step per
+// frame.  Prints one line of state statistics per reported frame and one
+// line per reload.
 //
-//   pbf_run <scenario.par> <frames> [report_every] [ramp]
+//   pbf_run <scenario.par> <frames> [report_every] [ramp] [--watch] [--fps N] [--log]
+//
+// ramp:    LoadForceMasks installs one collision quad, a ramp over the floor
+//          of the box (y = 0.4h at xMin up to 0.1 of the height at xMax),
+//          solid everywhere.
+// --watch: reload the scenario whenever it changes (the reference always
+//          watches); without it the file is read once.
+// --fps N: pace the loop to N frames per second, as a renderer would.
+// --log:   print the device event log after each reported frame.
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
+#include <list>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "Simulation.hpp"
 
-int main(int argc, char** argv) {
-    if (argc < 3) {
-        std::fprintf(stderr, "usage: %s <scenario.par> <frames> [report_every] [ramp]\n", argv[0]);
-        return 2;
-    }
-    std::ifstream f(argv[1]);
-    if (!f) {
-        std::fprintf(stderr, "cannot open %s\n", argv[1]);
-        return 2;
-    }
+namespace {
+
+std::string read_file(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open " + path);
     std::stringstream ss;
     ss << f.rdbuf();
+    return ss.str();
+}
+
+void install_ramp(pbf_host::Simulation& sim) {
+    const pbf_params& P = pbf_host::Params;
+    pbf_quad q{};
+    q.origin[0] = P.xMin;
+    q.origin[1] = P.yMin + 0.4f * P.h;
+    q.origin[2] = P.zMin;
+    q.e0[2] = P.zMax - P.zMin;
+    q.e1[0] = P.xMax - P.xMin;
+    q.e1[1] = 0.1f * (P.yMax - P.yMin);
+    q.mask_h = 1;
+    sim.LoadForceMasks({1u}, 1, 1, {q}, 0.0f);
+}
+
+const char* event_name(uint32_t code) {
+    switch (code) {
+        case PBF_LOG_FRIENDS_OVERFLOW: return "friends_overflow";
+        case PBF_LOG_NAN: return "nan";
+        case PBF_LOG_SORT_TIMEOUT: return "sort_timeout";
+        case PBF_LOG_INTERNAL: return "internal";
+        default: return "unknown";
+    }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s <scenario.par> <frames> [report_every] [ramp] [--watch] [--fps N] [--log]\n",
+                     argv[0]);
+        return 2;
+    }
+    const std::string path = argv[1];
     const int frames = std::atoi(argv[2]);
-    const int every = argc > 3 ? std::atoi(argv[3]) : 10;
+    int every = 10;
+    bool ramp = false, watch = false, log = false;
+    double fps = 0.0;
+    for (int a = 3; a < argc; ++a) {
+        const std::string s = argv[a];
+        if (s == "ramp")
+            ramp = true;
+        else if (s == "--watch")
+            watch = true;
+        else if (s == "--log")
+            log = true;
+        else if (s == "--fps" && a + 1 < argc)
+            fps = std::atof(argv[++a]);
+        else
+            every = std::max(1, std::atoi(argv[a]));
+    }
     try {
-        pbf_host::LoadParameters(ss.str());
         pbf_host::Simulation sim;
-        sim.InitBuffers();
-        sim.InitCells();
-        if (!sim.InitKernels()) return 1;
-        if (argc > 4 && std::string(argv[4]) == "ramp") {
-            const pbf_params& P = pbf_host::Params;
-            pbf_quad q{};
-            q.origin[0] = P.xMin;
-            q.origin[1] = P.yMin + 0.4f * P.h;
-            q.origin[2] = P.zMin;
-            q.e0[2] = P.zMax - P.zMin;
-            q.e1[0] = P.xMax - P.xMin;
-            q.e1[1] = 0.1f * (P.yMax - P.yMin);
-            q.mask_h = 1;
-            sim.LoadForceMasks({1u}, 1, 1, {q}, 0.0f);
-        }
+        std::list<std::pair<std::string, time_t>> tracked = {{path, 0}};
+        uint32_t prev_count = 0;
+        bool kernels_ok = false;
         std::vector<float> x, v;
+        auto next = std::chrono::steady_clock::now();
         for (int frame = 1; frame <= frames; ++frame) {
+            // src/Runner.cpp:43-77 (the first call sees the file as changed)
+            if ((frame == 1 || watch) && pbf_host::DetectResourceChanges(tracked)) {
+                pbf_host::LoadParameters(read_file(path));
+                const bool reset = frame == 1 || pbf_host::Params.particleCount != prev_count ||
+                                   pbf_host::Params.resetSimOnChange;
+                if (reset) {
+                    prev_count = pbf_host::Params.particleCount;
+                    sim.InitBuffers();
+                }
+                sim.InitCells();
+                if (ramp) install_ramp(sim);
+                kernels_ok = sim.InitKernels();
+                if (frame > 1) std::printf("reload frame %d reset %d ok %d\n", frame, reset ? 1 : 0, kernels_ok ? 1 : 0);
+                std::fflush(stdout);
+            }
+            if (!kernels_ok) continue;
             for (uint32_t s = 0; s < pbf_host::Params.subSteps; ++s) sim.Step();
             if (frame % every == 0 || frame == frames) {
                 sim.GetState(&x, &v);
@@ -65,6 +127,19 @@ int main(int argc, char** argv) {
                 }
                 std::printf("frame %d n %zu ke %.6e ymin %.5f ymax %.5f finite %d\n", frame, n, ke, ymin, ymax,
                             finite ? 1 : 0);
+                if (log) {
+                    std::vector<pbf_log_msg> msgs;
+                    uint64_t lost = 0;
+                    sim.ReadLog(&msgs, &lost);
+                    for (const pbf_log_msg& m : msgs)
+                        std::printf("event %s %u %u %u\n", event_name(m.code), m.a, m.b, m.c);
+                    if (lost) std::printf("event lost %llu\n", (unsigned long long)lost);
+                }
+                std::fflush(stdout);
+            }
+            if (fps > 0.0) {
+                next += std::chrono::microseconds((long long)(1e6 / fps));
+                std::this_thread::sleep_until(next);
             }
         }
         sim.UpdateTimings();

@@ -183,6 +183,7 @@ _SIGNATURES = {
     "pbf_friend_circle_thresholds": (ctypes.c_int, [ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "pbf_copy_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "pbf_set_state": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32]),
+    "pbf_update_params": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(PbfParams)]),
     "pbf_set_collision_quads": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(PbfQuad), ctypes.c_uint32, _UP,
                                                ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]),
     "pbf_step": (ctypes.c_int, [_VP, ctypes.c_uint32]),

@@ -62,6 +62,15 @@ class Simulation:
         vv = None if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 4)
         check(lib().pbf_set_state(self._ctx, _ptr(x), _ptr(vv), x.shape[0]), self._ctx)
 
+    def update_params(self, params: PbfParams, scene: int = 0):
+        """pbf_update_params: the scene's parameters for the next step (a
+        reloaded scenario); particleCount, gridBufSize and simIterations must
+        stay."""
+        self._ensure()
+        check(lib().pbf_update_params(self._ctx, int(scene), params), self._ctx)
+        if scene == 0 and not hasattr(self, "params_list"):
+            self.params = params.copy()
+
     def set_collision_quads(self, quads, mask=None, edge_base: float = 0.0, scene: int = 0):
         """pbf_set_collision_quads: the scene's collision quads (PbfQuad or
         (origin, e0, e1, mask_y, mask_h) each), applied after the walls, and

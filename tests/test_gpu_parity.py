@@ -285,6 +285,33 @@ def test_collision_quads_are_validated():
     sim.close()
 
 
+def test_update_params_matches_a_fresh_oracle():
+    # a reloaded scenario: after 4 steps the context takes new parameters; the
+    # next 4 steps equal an oracle started from the same state with them
+    scene = pbf_amd.config("dam4k", seed=6)
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 4)
+    p2 = scene.params.copy()
+    p2.garvity = 4.0
+    p2.viscosityFactor *= 3.0
+    p2.h = 0.105
+    p2.timeStep = 0.007
+    pbf_amd.lib().pbf_params_finalize(p2)
+    sim.update_params(p2)
+    st = ref.get_state()
+    ref2 = pbf_oracle.OracleSim(pbf_oracle.cfg_from_params(p2, scene.options))
+    ref2.set_state(st["x"], st["v"])
+    advance(sim, ref2, 4)
+    check_structure(sim, ref2)
+    check_state(sim, ref2, pbf_amd.Scene(p2, scene.options, scene.x, scene.v))
+    p3 = p2.copy()
+    p3.particleCount += 1
+    with pytest.raises(pbf_amd.PbfError) as e:
+        sim.update_params(p3)
+    assert e.value.code == pbf_amd._lib.PBF_E_INVALID
+    sim.close()
+
+
 def test_deterministic_repeat():
     scene = pbf_amd.config("dam16k", seed=4)
     outs = []

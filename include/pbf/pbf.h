@@ -241,6 +241,15 @@ int pbf_friend_circle_thresholds(float h, float *t4);
  * rocprofv3 FETCH_SIZE / WRITE_SIZE counters (kernel k_copy_probe). */
 int pbf_copy_probe(uint64_t bytes, uint32_t reps, double *gbps);
 
+/* Compares, on the device, the square root and reciprocal the sweeps use
+ * (hardware estimate plus one FMA correction) with IEEE sqrtf and 1.0f/x:
+ * the square root for every binary32 input from 2^-96 (below it a pair
+ * counts as coincident), the reciprocal for every normal input whose
+ * reciprocal is normal.  Counts of mismatches and the first mismatching bit
+ * pattern of each (0 if none).  No counterpart in the reference; it backs
+ * the bit-for-bit parity of the sweeps with the IEEE oracle. */
+int pbf_check_fast_math(uint64_t *sqrt_bad, uint64_t *rcp_bad, uint32_t *first_sqrt, uint32_t *first_rcp);
+
 void pbf_destroy(pbf_ctx *ctx);
 
 /* Message of the last failing call on ctx (or of the last pbf_create

@@ -78,6 +78,13 @@ template <> struct K<float> { static float c(float f) { return f; } };
 template <> struct K<Dual<double>> { static Dual<double> c(float f) { return {(double)f, 0.0}; } };
 template <> struct K<Dual<float>> { static Dual<float> c(float f) { return {f, 0.0}; } };
 
+// |r| from r^2 as the oracle's sweeps take it: 0 below 2^-96 (a coincident
+// pair, no direction), else the square root.
+template <class T>
+inline T pair_rl(T r2) {
+    return val(r2) >= 0x1p-96 ? t_sqrt(r2) : K<T>::c(0.0f);
+}
+
 struct Frozen {
     uint32_t n, iters, steps;
     const uint32_t* perm;      // steps x n
@@ -216,10 +223,10 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                     const T r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
                     const T r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
                     if (val(r2) < val(h2)) {
-                        const T rl = t_sqrt(r2);
+                        const T rl = pair_rl(r2);
                         const T hr = h - rl;
                         const T sp = spiky * hr * hr;
-                        const T inv = val(r2) > 0.0 ? one / rl : zero;
+                        const T inv = val(rl) > 0.0 ? one / rl : zero;
                         const T t = sp * inv;
                         const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                         const T d = h2 - r2;
@@ -245,9 +252,9 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                     const T r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
                     const T r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
                     if (val(r2) < val(h2)) {
-                        const T rl = t_sqrt(r2);
+                        const T rl = pair_rl(r2);
                         const T hr = h - rl;
-                        const T inv = val(r2) > 0.0 ? one / rl : zero;
+                        const T inv = val(rl) > 0.0 ? one / rl : zero;
                         const T t = inv * hr * hr;
                         const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                         const T d = h2 - r2;
@@ -306,9 +313,9 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                 if (val(r2) < val(h2) && fabs(val(rho[j])) > (double)1e-8f) {
                     const T u[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
                     const T d = h2 - r2;
-                    const T rl = t_sqrt(r2);
+                    const T rl = pair_rl(r2);
                     const T hr = h - rl;
-                    const T inv = val(r2) > 0.0 ? one / rl : zero;
+                    const T inv = val(rl) > 0.0 ? one / rl : zero;
                         const T t = inv * hr * hr;
                         const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                     o[0] += u[1] * gg[2] - u[2] * gg[1];
@@ -335,13 +342,13 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                 const T r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
                 const T r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
                 if (val(r2) < val(h2) && fabs(val(rho[j])) > (double)1e-8f) {
-                    const T rl = t_sqrt(r2);
+                    const T rl = pair_rl(r2);
                     const T hr = h - rl;
-                    const T inv = val(r2) > 0.0 ? one / rl : zero;
+                    const T inv = val(rl) > 0.0 ? one / rl : zero;
                         const T t = inv * hr * hr;
                         const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const T* oj = om[j].a;
-                    const T wl = t_sqrt(oj[0] * oj[0] + oj[1] * oj[1] + oj[2] * oj[2]);
+                    const T wl = pair_rl(oj[0] * oj[0] + oj[1] * oj[1] + oj[2] * oj[2]);
                     eta[0] += wl * gg[0]; eta[1] += wl * gg[1]; eta[2] += wl * gg[2];
                 }
             });

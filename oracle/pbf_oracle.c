@@ -188,6 +188,13 @@ void pbfo_cells(const uint32_t *ks, uint32_t n, uint32_t *cells)
 
 static inline float dot3(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
+/* |r| from r^2 for the sweeps: a pair with r^2 < 2^-96 (|r| < 2^-48) is
+ * coincident, with length and direction 0 (h - |r| rounds to h there); the
+ * same rule gives |omega_j| from its square.  The kernels evaluate the
+ * square root and reciprocal as a hardware estimate plus one correction,
+ * equal to sqrtf and 1.0f/x on exactly this domain. */
+static inline float pair_rl(float r2) { return r2 >= 0x1p-96f ? sqrtf(r2) : 0.0f; }
+
 static inline void cross3(const float *a, const float *b, float *o)
 {
     o[0] = a[1] * b[2] - a[2] * b[1];
@@ -263,10 +270,10 @@ void pbfo_scaling(const pbfo_cfg *c, const float *xs, const uint32_t *fr, float 
                 const float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
                 const float r2 = dot3(r, r);
                 if (r2 < h2) {
-                    const float rl = sqrtf(r2);
+                    const float rl = pair_rl(r2);
                     const float hr = h - rl;
                     const float s = spiky * hr * hr;
-                    const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 at r = 0 */
+                    const float inv = rl > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 when coincident */
                     const float t = s * inv;
                     const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const float d = h2 - r2;
@@ -422,9 +429,9 @@ void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const u
                 const float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
                 const float r2 = dot3(r, r);
                 if (r2 < h2) {
-                    const float rl = sqrtf(r2);
+                    const float rl = pair_rl(r2);
                     const float hr = h - rl;
-                    const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 at r = 0 */
+                    const float inv = rl > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 when coincident */
                     const float t = inv * hr * hr;
                     const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const float d = h2 - r2;
@@ -515,9 +522,9 @@ void pbfo_viscosity(const pbfo_cfg *c, const float *xs, const float *v_in, const
                 if (r2 < h2 && fabsf(pj[3]) > 1e-8f) {
                     const float vij[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
                     const float d = h2 - r2;
-                    const float rl = sqrtf(r2);
+                    const float rl = pair_rl(r2);
                     const float hr = h - rl;
-                    const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 at r = 0 */
+                    const float inv = rl > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 when coincident */
                     const float t = inv * hr * hr;
                     const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     float cr[3];
@@ -560,13 +567,13 @@ void pbfo_vorticity(const pbfo_cfg *c, const float *xs, float *v, const float *o
                 const float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
                 const float r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
                 if (r2 < h2 && fabsf(pj[3]) > 1e-8f) {
-                    const float rl = sqrtf(r2);
+                    const float rl = pair_rl(r2);
                     const float hr = h - rl;
-                    const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 at r = 0 */
+                    const float inv = rl > 0.0f ? 1.0f / rl : 0.0f;   /* r/|r| = r * (1/|r|); 0 when coincident */
                     const float t = inv * hr * hr;
                     const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const float *oj = F4(omega, j);
-                    const float wl = sqrtf(dot3(oj, oj));
+                    const float wl = pair_rl(dot3(oj, oj));
                     eta[0] += wl * g[0]; eta[1] += wl * g[1]; eta[2] += wl * g[2];
                 }
             }

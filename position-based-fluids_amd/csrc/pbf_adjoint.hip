@@ -74,8 +74,9 @@ __device__ __forceinline__ Pair pair_of(const float4& a, const float4& b, const 
     g.r = {a.x - b.x, a.y - b.y, a.z - b.z};
     g.r2 = dot(g.r, g.r);
     g.in = g.r2 < c.h2;
-    g.q = sqrtf(g.r2);
-    g.inv = g.r2 > 0.0f ? 1.0f / g.q : 0.0f;   // direction of a coincident pair: 0
+    const PairLen L = pair_len(g.r2);   // as the forward sweeps: 0 and 0 for a coincident pair
+    g.q = L.rl;
+    g.inv = L.inv;
     g.hr = c.h - g.q;
     g.d = c.h2 - g.r2;
     return g;
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b1(const SceneConsts* __re
             [&](const G2& t) {
                 const Pair g = pair_of(p, t.a, c);
                 const V3 oj = v3(t.b);
-                eta += sel(g.in && fabsf(t.a.w) > 1e-8f, psi(g) * sqrtf(dot(oj, oj)));
+                eta += sel(g.in && fabsf(t.a.w) > 1e-8f, psi(g) * norm_of_sq(dot(oj, oj)));
             });
         eta = eta * -c.spiky;
         const float l = sqrtf(dot(eta, eta));
@@ -369,11 +370,11 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b2(
         [&](const G2& t) {
             const Pair g = pair_of(p, t.a, c);
             const V3 oj = v3(t.b);
-            gP += sel(g.in && fabsf(t.a.w) > 1e-8f, jpsi(g, zm, c) * sqrtf(dot(oj, oj)));
+            gP += sel(g.in && fabsf(t.a.w) > 1e-8f, jpsi(g, zm, c) * norm_of_sq(dot(oj, oj)));
         });
     if (fabsf(p.w) > 1e-8f) {
         const V3 omm = v3(om[m]);
-        const float wl = sqrtf(dot(omm, omm));
+        const float wl = norm_of_sq(dot(omm, omm));
         float gs = 0.0f;
         walk_quads(
             m, t05(tcnt[m]), tquads, [&](uint32_t i) { return G2{P[i], z[i]}; },

@@ -112,6 +112,46 @@ __device__ __forceinline__ float noise_sin(float x) {
     return (((int32_t)k) & 1) ? -p : p;
 }
 
+// Square root and reciprocal of a normal binary32 number, correctly rounded,
+// from the hardware estimates (error below one ulp).  Square root: sqrt(x)
+// rounds to one of s-, s, s+ around the estimate s; s- is the rounding when
+// sqrt(x) lies below the midpoint of s- and s, i.e. when x <= s- * s (the
+// products of two floats and x are multiples of a common unit far above the
+// square of the half-ulp, so comparing with s- * s instead of the midpoint's
+// square decides the same way), and likewise s+ when x > s+ * s; each
+// comparison is the sign of one exact-signed FMA residual.  Reciprocal: one
+// Newton-Raphson step y + y (1 - x y) with FMA.  Both are checked against
+// IEEE sqrtf and 1.0f / x on the device for every normal input
+// (pbf_check_fast_math).
+__device__ __forceinline__ float sqrt_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float below = __uint_as_float(__float_as_uint(s) - 1u);
+    const float above = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = fmaf(-below, s, x) <= 0.0f ? below : s;
+    r = fmaf(-above, s, x) > 0.0f ? above : r;
+    return r;
+}
+__device__ __forceinline__ float rcp_normal(float x) {
+    const float y = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, y, 1.0f), y, y);
+}
+
+// A pair whose binary32 r^2 is below 2^-96 (|r| < 2^-48) is coincident: its
+// length and direction are 0 (h - |r| rounds to h there anyway).  Above it
+// the square root estimate is within an ulp, so |r| and 1/|r| are the IEEE
+// values.  A length from a squared norm follows the same rule.
+constexpr float kTinySq = 0x1p-96f;
+struct PairLen {
+    float rl, inv;   // |r|, 1/|r|
+};
+__device__ __forceinline__ PairLen pair_len(float r2) {
+    const bool ok = r2 >= kTinySq;
+    const float rl = sqrt_normal(r2);
+    const float inv = rcp_normal(rl);
+    return {ok ? rl : 0.0f, ok ? inv : 0.0f};
+}
+__device__ __forceinline__ float norm_of_sq(float s2) { return s2 >= kTinySq ? sqrt_normal(s2) : 0.0f; }
+
 __device__ __forceinline__ int circle_count(uint32_t packed, int q) { return (packed >> (6 * q)) & 63; }
 
 // Number of friend circles a sweep processes.  The reference skips circle q

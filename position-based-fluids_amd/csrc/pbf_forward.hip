@@ -543,11 +543,10 @@ struct LambdaAcc {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         const bool in = r2 < c.h2;
-        const float rl = sqrtf(r2);
-        const float hr = c.h - rl;
+        const PairLen L = pair_len(r2);   // r/|r| = r * (1/|r|); 0 for a coincident pair
+        const float hr = c.h - L.rl;
         const float s = c.spiky * hr * hr;
-        const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;   // r/|r| = r * (1/|r|); 0 at r = 0
-        const float t = s * inv;
+        const float t = s * L.inv;
         const float ax = in ? rx * t : 0.0f, ay = in ? ry * t : 0.0f, az = in ? rz * t : 0.0f;
         const float d = in ? c.h2 - r2 : 0.0f;
         dens += d * d * d;
@@ -592,10 +591,9 @@ struct DeltaAcc {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         const bool in = r2 < c.h2;
-        const float rl = sqrtf(r2);
-        const float hr = c.h - rl;
-        const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;
-        const float t = inv * hr * hr;
+        const PairLen L = pair_len(r2);
+        const float hr = c.h - L.rl;
+        const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
         const float d = c.h2 - r2;
         const float p6 = d * d * d;
@@ -663,10 +661,9 @@ struct ViscAcc {
         const bool in = r2 < c.h2 && fabsf(pj.w) > 1e-8f;
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float d = c.h2 - r2;
-        const float rl = sqrtf(r2);
-        const float hr = c.h - rl;
-        const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;
-        const float t = inv * hr * hr;
+        const PairLen L = pair_len(r2);
+        const float hr = c.h - L.rl;
+        const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
         ox += in ? uy * az - uz * ay : 0.0f;
         oy += in ? uz * ax - ux * az : 0.0f;
@@ -710,12 +707,11 @@ struct VortAcc {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         const bool in = r2 < c.h2 && fabsf(pj.w) > 1e-8f;
-        const float rl = sqrtf(r2);
-        const float hr = c.h - rl;
-        const float inv = r2 > 0.0f ? 1.0f / rl : 0.0f;
-        const float t = inv * hr * hr;
+        const PairLen L = pair_len(r2);
+        const float hr = c.h - L.rl;
+        const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
-        const float wl = sqrtf(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
+        const float wl = norm_of_sq(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
         ex += in ? wl * ax : 0.0f;
         ey += in ? wl * ay : 0.0f;
         ez += in ? wl * az : 0.0f;

@@ -182,6 +182,7 @@ _SIGNATURES = {
                                     ctypes.POINTER(ctypes.c_uint64)]),
     "pbf_friend_circle_thresholds": (ctypes.c_int, [ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "pbf_copy_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    "pbf_check_fast_math": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _UP, _UP]),
     "pbf_set_state": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32]),
     "pbf_update_params": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(PbfParams)]),
     "pbf_set_collision_quads": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(PbfQuad), ctypes.c_uint32, _UP,

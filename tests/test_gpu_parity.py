@@ -481,3 +481,13 @@ def test_staged_friends_build_matches(size):
     check_structure(sim, ref)
     check_state(sim, ref, scene)
     sim.close()
+
+
+def test_fast_sqrt_and_reciprocal_are_ieee():
+    # every normal binary32 input: the kernels' square root and reciprocal
+    # equal sqrtf and 1.0f / x
+    import ctypes
+    a, b = ctypes.c_uint64(), ctypes.c_uint64()
+    fa, fb = ctypes.c_uint32(), ctypes.c_uint32()
+    assert pbf_amd.lib().pbf_check_fast_math(ctypes.byref(a), ctypes.byref(b), ctypes.byref(fa), ctypes.byref(fb)) == 0
+    assert (a.value, b.value) == (0, 0), (a.value, hex(fa.value), b.value, hex(fb.value))

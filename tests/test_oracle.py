@@ -577,13 +577,16 @@ def test_nan_particle_is_nobody_s_friend(oracle):
     assert idx[0, 0, 0] == 1 and idx[0, 0, 1] == 0
 
 
-def test_coincident_pair_has_no_direction(oracle):
-    """Two particles at one point: the pair counts in the density,
-    (h^2)^3 POLY6, but r/|r| is taken as 0, so its gradient terms vanish:
-    lambda = -C/eps and delta-p = 0 (the reference's 0/0 would give NaN)."""
+@pytest.mark.parametrize("sep", [0.0, 2.0 ** -50])
+def test_coincident_pair_has_no_direction(oracle, sep):
+    """Two particles at one point, or closer than 2^-48 (r^2 below 2^-96):
+    the pair counts in the density, (h^2)^3 POLY6, but r/|r| is taken as 0, so
+    its gradient terms vanish: lambda = -C/eps and delta-p = 0 (the
+    reference's 0/0 would give NaN)."""
     h = 0.1
     xs = np.zeros((2, 4), np.float32)
-    xs[:, :3] = (0.5, 0.5, 0.5)
+    xs[:, :3] = (0.0, 0.5, 0.5)
+    xs[1, 0] = np.float32(sep)
     fr = np.zeros(2 * 255, np.uint32)
     fr[0:2] = 1
     fr[5 * 2 + 0] = 1
@@ -603,3 +606,26 @@ def test_coincident_pair_has_no_direction(oracle):
     delta = np.full((2, 4), 7.0, np.float32)
     po.lib().pbfo_delta(ctypes.byref(c), po.ptr(xs), po.ptr(xs), po.ptr(fr), po.ptr(delta))
     assert delta[:, :3].tolist() == [[0.0, 0.0, 0.0], [0.0, 0.0, 0.0]]
+
+
+def test_pair_just_above_the_coincidence_bound_has_a_direction(oracle):
+    """At |r| = 2^-47 (r^2 = 2^-94) the pair has its direction: delta-p of
+    the two particles is equal and opposite along x and nonzero."""
+    h = 0.1
+    xs = np.zeros((2, 4), np.float32)
+    xs[:, :3] = (0.0, 0.25, 0.25)
+    xs[1, 0] = np.float32(-(2.0 ** -47))
+    fr = np.zeros(2 * 255, np.uint32)
+    fr[0:2] = 1
+    fr[5 * 2 + 0] = 1
+    fr[5 * 2 + 1] = 0
+    p = pbf_amd.physical_params(2, h, (0, 0, 0), (1, 1, 1))
+    c = po.cfg_from_params(p, pbf_amd.default_options(walls=False))
+    rho = np.zeros(2, np.float32)
+    lam = np.zeros(2, np.float32)
+    po.lib().pbfo_scaling(ctypes.byref(c), po.ptr(xs), po.ptr(fr), po.ptr(rho), po.ptr(lam))
+    xs[:, 3] = lam
+    delta = np.zeros((2, 4), np.float32)
+    po.lib().pbfo_delta(ctypes.byref(c), po.ptr(xs), po.ptr(xs), po.ptr(fr), po.ptr(delta))
+    assert delta[0, 0] != 0.0 and delta[0, 0] == -delta[1, 0]
+    assert delta[0, 1] == 0.0 and delta[0, 2] == 0.0

@@ -20,9 +20,12 @@
 // bucket visits, i lists m in circle c iff m lists i in circle c: r^2 is
 // computed from an exactly antisymmetric difference and the 27-cell
 // neighbourhoods are symmetric.  So T_m = {i in m's full list, circle c :
-// c < circles processed by i}, built when the substep is recorded; drops and
-// duplicate buckets are detected and make the backward fail rather than
-// differentiate a different function.
+// c < circles processed by i}, built when the substep is recorded
+// (k_transpose).  A substep that dropped friends or reached a bucket twice
+// is transposed by searching instead (k_transpose_all: the candidates in m's
+// neighbour cells whose prefixes list m); the backward fails rather than
+// differentiate a different function if that search misses an entry or a
+// list outgrows its slot.
 //
 // Per-pair derivatives, r = p_i - p_j, q = |r|, d = h^2 - q^2:
 //   psi(r) = r (h-q)^2 / q            (the spiky direction without SPIKY)
@@ -45,6 +48,10 @@ namespace {
 constexpr int kAdjThreads = 256;   // workgroup of every adjoint kernel (block table bt[2])
 constexpr int kParts = 8;          // per-tile partial sums: [0] viscosity, [1] vorticity, [2..4] gravity
 constexpr uint32_t kDupFlag = 1u;  // a particle's 27-cell neighbourhood visits one bucket twice
+constexpr uint32_t kCapFlag = 2u;  // a transposed list would exceed the slot's 4 * kQuads entries
+// flags words: [0] kDupFlag | kCapFlag, [2..3] entries placed by k_transpose_all,
+// [4..5] entries of the prefixes it transposed (u64 each)
+constexpr int kFlagWords = 8;
 
 struct V3 {
     float x, y, z;
@@ -251,6 +258,99 @@ __global__ __launch_bounds__(kAdjThreads) void k_transpose(const SceneConsts* __
                     keys[nk++] = k;
                 }
         if (dup) atomicOr(flags, kDupFlag);
+    }
+}
+
+// Transposition of a substep whose lists need not be symmetric (friends
+// dropped at the 50 cap, or a bucket reached twice): T_m holds every i whose
+// processed prefix lists m, as often as it lists m, the 0.5-prefix entries
+// first.  A particle that lists m is within h of it, so it lies in one of m's
+// 27 neighbour cells: m scans their distinct buckets in order, keeps the
+// candidates that pass the friends build's own binary32 test (r^2 < h^2, r =
+// p_i - p_m) and counts m in each one's prefix.  Runs only when this
+// substep's dropped-friends snapshot moved (note[-1], note[0]) or a repeated
+// bucket was seen, and then replaces k_transpose's lists.  It adds the
+// entries it placed and the prefix lengths of all particles to flags[2..5];
+// the two agree unless a particle outside the neighbour cells listed one
+// (only possible at the rounding of a cell boundary), and a list longer
+// than the slot sets kCapFlag.
+__global__ __launch_bounds__(kAdjThreads) void k_transpose_all(
+    const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ xs,
+    const uint32_t* __restrict__ cnt, const uint4* __restrict__ sweep, const uint2* __restrict__ cells, uint32_t n,
+    uint32_t* __restrict__ tcnt, uint4* __restrict__ tquads, const unsigned long long* __restrict__ note,
+    uint32_t* flags) {
+    if (note[0] == note[-1] && !(flags[0] & kDupFlag)) return;
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
+    const SceneConsts& c = scenes[be.x];
+    const uint32_t m = be.y + threadIdx.x;
+    unsigned long long placed = 0, listed = 0;
+    if (m < be.z) {
+        const float4 p = xs[m];
+        int own;
+        circles_processed<3, 5>(cnt[m], &own);
+        listed = (unsigned long long)own;
+        const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
+        uint32_t keys[27];
+        int nk = 0;
+        for (int k = 0; k < 27; ++k) {
+            const uint32_t key = cell_hash(cx + k / 9 - 1, cy + (k / 3) % 3 - 1, cz + k % 3 - 1, c);
+            bool seen = false;
+            for (int t = 0; t < nk; ++t) seen |= keys[t] == key;
+            if (!seen && cells[key].x != kEmpty) keys[nk++] = key;
+        }
+        uint32_t acc[4] = {0u, 0u, 0u, 0u};
+        int na = 0, qo = 0, n05 = 0;
+        bool full = false;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int t = 0; t < nk; ++t) {
+                const uint2 b = cells[keys[t]];
+                const uint32_t e = min(b.y, n - 1);
+                for (uint32_t i = b.x; i <= e; ++i) {
+                    if (i == m) continue;
+                    const float4 q = xs[i];
+                    const float rx = q.x - p.x, ry = q.y - p.y, rz = q.z - p.z;
+                    if (!(rx * rx + ry * ry + rz * rz < c.h2)) continue;
+                    int m05, m06;
+                    const uint32_t pi = cnt[i];
+                    circles_processed<1, 2>(pi, &m05);
+                    circles_processed<3, 5>(pi, &m06);
+                    const int lo = pass == 0 ? 0 : m05, hi = pass == 0 ? m05 : m06;
+                    for (int k = lo & ~3; k < hi; k += 4) {
+                        const uint4 qd = sweep[sweep_quad(i, k >> 2)];
+                        const uint32_t v4[4] = {qd.x, qd.y, qd.z, qd.w};
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (k + u < lo || k + u >= hi || v4[u] != m) continue;
+                            if (4 * qo + na >= 4 * kQuads) {
+                                full = true;
+                                continue;
+                            }
+#pragma unroll
+                            for (int w = 0; w < 4; ++w) acc[w] = (w == na) ? i : acc[w];
+                            ++placed;
+                            if (++na == 4) {
+                                tquads[sweep_quad(m, qo++)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
+                                na = 0;
+                            }
+                        }
+                    }
+                }
+            }
+            if (pass == 0) n05 = 4 * qo + na;
+        }
+        if (na) tquads[sweep_quad(m, qo)] = make_uint4(acc[0], acc[1], acc[2], acc[3]);
+        tcnt[m] = (uint32_t)(4 * qo + na) | ((uint32_t)n05 << 16);
+        if (full) atomicOr(flags, kCapFlag);
+    }
+    // one add per wave
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        placed += __shfl_xor(placed, o, 64);
+        listed += __shfl_xor(listed, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && (placed | listed)) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(flags + 2), placed);
+        atomicAdd(reinterpret_cast<unsigned long long*>(flags + 4), listed);
     }
 }
 
@@ -674,7 +774,7 @@ struct pbf_adjoint {
     int* expo = nullptr;                   // per scene: state = true * 2^expo
     uint2* tile_range = nullptr;
     unsigned long long* notes = nullptr;   // C + 1 snapshots of the dropped-friends counter
-    uint32_t* flags = nullptr;
+    uint32_t* flags = nullptr;             // kFlagWords: dup / capacity flags, transposition counts
     bool have_forward = false;
     int recorded = -1;                     // segment whose substeps the slots hold
     double fwd_ms = 0.0, bwd_ms = 0.0, replay_ms = 0.0;
@@ -739,6 +839,9 @@ int tape_step(pbf_adjoint* a, const TapeSlot& t, unsigned long long* note) {
                            t.cnt, ctx->nb_list, reinterpret_cast<const uint32_t*>(t.quads),
                            reinterpret_cast<const uint2*>(ctx->cells), t.tcnt, t.tquads, a->flags);
         hipLaunchKernelGGL(k_note_drops, dim3(1), dim3(64), 0, ctx->stream, ctx->err, note);
+        hipLaunchKernelGGL(k_transpose_all, dim3(bt.count), dim3(kAdjThreads), 0, ctx->stream, ctx->sc_d, bt.d, t.P,
+                           t.cnt, t.quads, reinterpret_cast<const uint2*>(ctx->cells), ctx->n, t.tcnt, t.tquads, note,
+                           a->flags);
         e = hipGetLastError();
     }
     for (int k = 0; k < K && e == hipSuccess; ++k) {
@@ -782,27 +885,26 @@ int record_segment(pbf_adjoint* a, uint32_t g) {
     return PBF_OK;
 }
 
-// The recorded segment g is differentiable by the transposed lists: no
-// friend dropped and no bucket visited twice (checked on the host after the
-// device work has finished).
+// The transposed lists recorded so far are complete (checked on the host
+// after the device work has finished).
 int check_segment(pbf_adjoint* a, uint32_t g) {
     pbf_ctx* ctx = a->ctx;
-    const uint32_t len = std::min(a->T, (g + 1) * a->C) - g * a->C;
-    std::vector<unsigned long long> notes(len + 1);
-    uint32_t flags = 0;
+    uint32_t w[kFlagWords];
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    HIP_TRY(ctx, hipMemcpy(notes.data(), a->notes, 8 * (len + 1), hipMemcpyDeviceToHost));
-    HIP_TRY(ctx, hipMemcpy(&flags, a->flags, 4, hipMemcpyDeviceToHost));
-    for (uint32_t t = 0; t < len; ++t)
-        if (notes[t + 1] != notes[t])
-            return fail(ctx, PBF_E_STATE,
-                        "substep %u dropped %llu friends (50 per circle): its friend lists are not symmetric, "
-                        "which the adjoint requires",
-                        g * a->C + t, notes[t + 1] - notes[t]);
-    if (flags & kDupFlag)
+    HIP_TRY(ctx, hipMemcpy(w, a->flags, sizeof w, hipMemcpyDeviceToHost));
+    unsigned long long placed, listed;
+    memcpy(&placed, w + 2, 8);
+    memcpy(&listed, w + 4, 8);
+    if (w[0] & kCapFlag)
         return fail(ctx, PBF_E_STATE,
-                    "a particle's 27 neighbour cells share a hash bucket (gridBufSize folds them); the adjoint "
-                    "requires each bucket visited once -- use a power-of-two gridBufSize >= 64");
+                    "segment %u: a particle is listed by more than %d others (a dense cluster at the 50-per-circle "
+                    "cap); its transposed list does not fit",
+                    g, 4 * kQuads);
+    if (placed != listed)
+        return fail(ctx, PBF_E_STATE,
+                    "segment %u: %llu of %llu friend entries have no transposed entry (a friend outside the "
+                    "27 neighbour cells)",
+                    g, listed - placed, listed);
     return PBF_OK;
 }
 
@@ -972,7 +1074,7 @@ int pbf_adjoint_create(pbf_ctx* ctx, uint32_t substeps, uint32_t interval, pbf_a
     ADJ_TRY(pbf::aalloc(a, &a->expo, ctx->S));
     ADJ_TRY(pbf::aalloc(a, &a->tile_range, ctx->S));
     ADJ_TRY(pbf::aalloc(a, &a->notes, (size_t)a->C + 1));
-    ADJ_TRY(pbf::aalloc(a, &a->flags, 1));
+    ADJ_TRY(pbf::aalloc(a, &a->flags, pbf::kFlagWords));
     // tiles of the 256-particle block table per scene
     std::vector<uint2> tr(ctx->S);
     uint32_t tile = 0;
@@ -988,7 +1090,7 @@ int pbf_adjoint_create(pbf_ctx* ctx, uint32_t substeps, uint32_t interval, pbf_a
         return PBF_E_INVALID;
     }
     ADJ_TRY(hipMemcpy(a->tile_range, tr.data(), sizeof(uint2) * ctx->S, hipMemcpyHostToDevice));
-    ADJ_TRY(hipMemset(a->flags, 0, 4));
+    ADJ_TRY(hipMemset(a->flags, 0, 4 * pbf::kFlagWords));
 #undef ADJ_TRY
     *out = a;
     return PBF_OK;
@@ -1013,7 +1115,7 @@ int pbf_adjoint_forward(pbf_adjoint* a, const float* x0, const float* v0, float*
     }
     if (n == 0) return PBF_OK;
     hipLaunchKernelGGL(pbf::k_iota, pbf::grid_of(ctx->n), dim3(256), 0, ctx->stream, a->ids, ctx->n);
-    HIP_TRY(ctx, hipMemsetAsync(a->flags, 0, 4, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(a->flags, 0, 4 * pbf::kFlagWords, ctx->stream));
     for (uint32_t g = 0; g < a->segs; ++g) {
         HIP_TRY(ctx, hipMemcpyAsync(a->ckpt + (size_t)g * 2 * n, ctx->x, 16 * n, hipMemcpyDeviceToDevice,
                                     ctx->stream));

@@ -236,9 +236,10 @@ def test_torch_autograd_function(problem):
     assert coef.grad.device.type == "cpu" and n == x0.grad.shape[0]
 
 
-def test_dropped_friends_are_refused():
-    # 8x8x8 particles at 0.1h: every particle has far more than 50 friends in
-    # circle 0, so the friend lists are capped and not symmetric
+def test_dense_cluster_is_refused():
+    # 8x8x8 particles at 0.1h: every particle has all 511 others in circle 0
+    # and keeps the first 50 in scan order, so the first particles of the
+    # scan are listed by far more than a transposed list holds
     x = pbf_amd.dam_break_block(8, 8, 8, 0.1, seed=0, spacing=0.1)
     p = pbf_amd.physical_params(x.shape[0], 0.1, (0, 0, 0), (1.6, 1.6, 1.6))
     sim = pbf_amd.Simulation(p, pbf_amd.default_options())
@@ -247,5 +248,51 @@ def test_dropped_friends_are_refused():
     with pytest.raises(pbf_amd.PbfError) as e:
         roll.backward_raw(xT.clone(), None)
     assert e.value.code == pbf_amd._lib.PBF_E_STATE
+    assert "listed by more than" in str(e.value)
     roll.close()
     sim.close()
+
+
+def _asymmetric_problem(oracle, gbs=None, spacing=0.5, steps=6, seed=21):
+    """A 12x10x10 block with random velocities in a box twice its size:
+    compressed to `spacing` h (0.3: interior particles have 56 neighbours
+    in circle 0, so the 50 cap drops some and the lists are not symmetric),
+    or with a gridBufSize that folds neighbour cells into one bucket."""
+    h = 0.1
+    x = pbf_amd.dam_break_block(12, 10, 10, h, seed=seed, spacing=spacing)
+    ext = np.array([12, 10, 10]) * spacing * h
+    p = pbf_amd.physical_params(x.shape[0], h, (0, 0, 0), tuple(2 * ext), grid_buf_size=gbs)
+    v = np.zeros_like(x)
+    v[:, :3] = np.random.default_rng(seed).normal(0.0, 0.3, (x.shape[0], 3)).astype(np.float32)
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(), x, v)
+    return Problem(oracle, scene, steps=steps, interval=3, seed=3)
+
+
+@pytest.mark.parametrize("case", ["dropped_friends", "folded_buckets"])
+def test_gradients_with_asymmetric_lists(oracle, case):
+    # the transposed lists come from the search (k_transpose_all): the gradient
+    # still matches the binary32 JVP of the same frozen structure, to 1e-4
+    # (measured: 3e-6 and 2e-7; the symmetric transposition alone is off by
+    # 4e-3 and 2e-1 here)
+    tol = 1e-4
+    if case == "dropped_friends":
+        p = _asymmetric_problem(oracle, spacing=0.3)
+        assert p.sim.error_flags()[1] > 0
+    else:
+        p = _asymmetric_problem(oracle, gbs=97)
+    try:
+        assert np.array_equal(p.xT, p.fr.xT32)
+        rng = np.random.default_rng(4)
+        dv = rng.normal(0, 1, (p.scene.n, 3))
+        adj = float(np.sum(p.gv0[:, :3] * dv))
+        _, _, dxT, dvT = p.fr.jvp32(dv0=dv)
+        jvp = p.dL(dxT, dvT)
+        assert abs(adj - jvp) <= tol * abs(jvp), (adj, jvp)
+        for k in (0, 3):
+            dc = np.zeros(5)
+            dc[k] = 1.0
+            _, _, dxT, dvT = p.fr.jvp32(dcoef=dc)
+            jvp = p.dL(dxT, dvT)
+            assert abs(p.gc[0, k] - jvp) <= tol * abs(jvp), (k, p.gc[0, k], jvp)
+    finally:
+        p.close()

@@ -41,7 +41,7 @@ STAGE_BYTES = {
 # 1,048,576-particle dam break (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a
 # plain 256 MiB copy gives the two counters), averaged over the K launches of
 # a substep; profiles/pmc_traffic_dam1m.md holds the counts and commands.
-PMC_TRAFFIC_BYTES_PER_LAUNCH = {"computeDelta": 185.9e6}
+PMC_TRAFFIC_BYTES_PER_LAUNCH = {"computeDelta": 183.2e6, "computeScaling": 179.4e6}
 
 
 def kernel_bytes_per_particle(name: str, iters: int, passes: int) -> float | None:
@@ -124,18 +124,27 @@ def batched_gain(steps: int, warmup: int, seed: int, tune: bool) -> dict:
         if sizes:
             sim.set_workgroup_sizes(sizes)
         singles.append(sim)
-    for _ in range(warmup):
+    def loop_time(graph: bool) -> float:
         for sim in singles:
-            sim.Step()
-    for sim in singles:
-        sim.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+            sim.set_graph(graph)
+        for _ in range(warmup):
+            for sim in singles:
+                sim.Step()
         for sim in singles:
-            sim.Step()
-    for sim in singles:
-        sim.sync()
-    t_loop = time.perf_counter() - t0
+            sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for sim in singles:
+                sim.Step()
+        for sim in singles:
+            sim.sync()
+        return time.perf_counter() - t0
+
+    # the loop over the single-scene path, with eager launches and with each
+    # context's substeps replayed as graphs; the gain is against the faster
+    t_eager = loop_time(False)
+    t_graph = loop_time(True)
+    t_loop = min(t_eager, t_graph)
     for sim in singles:
         sim.close()
     return {
@@ -144,6 +153,8 @@ def batched_gain(steps: int, warmup: int, seed: int, tune: bool) -> dict:
         "steps": steps,
         "batched_particle_substeps_per_s": total * steps / t_batch,
         "looped_particle_substeps_per_s": total * steps / t_loop,
+        "looped_eager_particle_substeps_per_s": total * steps / t_eager,
+        "looped_graph_particle_substeps_per_s": total * steps / t_graph,
         "gain": t_loop / t_batch,
         "workgroup_sizes": sizes,
     }
@@ -327,17 +338,28 @@ def main() -> int:
     sim.run(args.warmup)
     # tuned on the warmed-up state, which autotune restores afterwards
     sizes = sim.autotune(2) if not args.no_autotune else sim.stats()["workgroup_size"]
+    # the timed region: substeps replayed as HIP graphs, no events between launches
+    sim.set_graph(True)
+    sim.run(2)
     sim.sync()
-    sim.set_profiling(True)
-    sim.sync()
+    start = sim.get_state()
     t0 = time.perf_counter()
     sim.run(args.steps)
     sim.sync()
     el = time.perf_counter() - t0
+    # the same substeps again from the same state (bitwise the same work), with
+    # HIP events around every launch on the context's stream: the per-stage
+    # breakdown and the roofline kernel's launch durations
+    sim.set_state(start["x"], start["v"])
+    sim.set_profiling(True)
+    sim.reset_timings()
+    sim.run(args.steps)
+    sim.sync()
     timings = sim.timings()
     sim.set_profiling(False)
     flags, dropped = sim.error_flags()
     sim.close()
+    prof_steps = args.steps
 
     value = n * args.steps / el
     _progress(f"forward: {value:.4g} particle-substeps/s")
@@ -361,7 +383,7 @@ def main() -> int:
                            for name, t in launches) / max(tot_calls, 1)
     avg_ms = tot_ms / max(tot_calls, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    step_ms = {k: round(v / args.steps, 4) for k, v in sorted(fam_ms.items(), key=lambda kv: -kv[1])}
+    step_ms = {k: round(v / prof_steps, 4) for k, v in sorted(fam_ms.items(), key=lambda kv: -kv[1])}
 
     out = {
         "metric": METRIC,

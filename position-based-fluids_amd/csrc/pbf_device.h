@@ -269,12 +269,53 @@ __device__ __forceinline__ void walk_quads(uint32_t i, int m, const uint4* __res
     }
 }
 
-template <int L_NUM, int L_DEN, class Load, class Add>
+// The same walk one quad deeper: the gathers of quad q + 1 are issued before
+// quad q is folded in, so a wave keeps its next gathers in flight while it
+// computes.  Entries past m are replaced by i, so every gather stays in
+// bounds; only the first m are added, in list order.
+template <class Load, class Add>
+__device__ __forceinline__ void walk_quads_ahead(uint32_t i, int m, const uint4* __restrict__ quads, Load&& load,
+                                                 Add&& add) {
+    if (m <= 0) return;
+    const int nq = (m + 3) >> 2;
+    auto own = [&](uint4 u, int k) -> uint4 {
+        const int b = 4 * k;
+        return make_uint4(b < m ? u.x : i, b + 1 < m ? u.y : i, b + 2 < m ? u.z : i, b + 3 < m ? u.w : i);
+    };
+    const uint4 first = own(quads[sweep_quad(i, 0)], 0);
+    auto g0 = load(first.x);
+    auto g1 = load(first.y);
+    auto g2 = load(first.z);
+    auto g3 = load(first.w);
+    uint4 next = quads[sweep_quad(i, 1)];
+    for (int q = 0; q < nq; ++q) {
+        const uint4 u = own(next, q + 1);
+        const auto h0 = load(u.x);
+        const auto h1 = load(u.y);
+        const auto h2 = load(u.z);
+        const auto h3 = load(u.w);
+        next = quads[sweep_quad(i, min(q + 2, kQuads - 1))];   // a particle's 63 quads
+        const int b = 4 * q;
+        add(g0);
+        if (b + 1 < m) add(g1);
+        if (b + 2 < m) add(g2);
+        if (b + 3 < m) add(g3);
+        g0 = h0;
+        g1 = h1;
+        g2 = h2;
+        g3 = h3;
+    }
+}
+
+template <int L_NUM, int L_DEN, bool AHEAD = false, class Load, class Add>
 __device__ __forceinline__ void walk_friends(uint32_t i, uint32_t packed, const uint4* __restrict__ quads, Load&& load,
                                              Add&& add) {
     int m;
     (void)circles_processed<L_NUM, L_DEN>(packed, &m);
-    walk_quads(i, m, quads, load, add);
+    if (AHEAD)
+        walk_quads_ahead(i, m, quads, load, add);
+    else
+        walk_quads(i, m, quads, load, add);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the chip; this

@@ -570,7 +570,7 @@ __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ s
     if (i >= be.z) return;
     const float4 p = xs[i];
     LambdaAcc a;
-    walk_friends<3, 5>(
+    walk_friends<3, 5, true>(
         i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; }, [&](const float4& q) { a.add(p, q, c); });
     const float dens = a.dens * c.poly6;
     rho[i] = dens;
@@ -620,7 +620,7 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
     if (i >= be.z) return;
     const float4 p = xs[i];
     DeltaAcc a;
-    walk_friends<1, 2>(
+    walk_friends<1, 2, true>(
         i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; }, [&](const float4& q) { a.add(p, q, c); });
     float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
     float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
@@ -688,7 +688,7 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
     const float4 p = xs[i];
     const float4 vi = v_in[i];
     ViscAcc a;
-    walk_friends<1, 2>(
+    walk_friends<1, 2, true>(
         i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], v_in[j]}; },
         [&](const Pair4& t) { a.add(p, vi, t.a, t.b, c); });
     float4 vo;

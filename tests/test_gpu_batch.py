@@ -89,3 +89,26 @@ def test_batch_per_scene_quads(graph):
     q0, m0 = box_quads(scenes[0].params, mask_seed=1)
     q2, m2 = box_quads(scenes[2].params, plate_x=0.3, mask_seed=2)
     check_batch(scenes, 6, graph=graph, quads=[(q0, m0, 0.002), None, (q2[1:], m2, 0.0)])
+
+
+def test_full_ensemble_equals_its_scenes():
+    # the bench's 64-scene ensemble (about 2.3M particles): the batch equals
+    # each scene stepped alone, bit for bit
+    scenes = pbf_amd.ensemble(64, seed=0)
+    steps = 3
+    batch = pbf_amd.BatchSimulation([s.params for s in scenes], [s.options for s in scenes])
+    batch.set_state(np.concatenate([s.x for s in scenes]), np.concatenate([s.v for s in scenes]))
+    batch.set_graph(True)
+    batch.run(steps)
+    got = batch.get_state()
+    off = batch.offsets
+    batch.close()
+    for k, s in enumerate(scenes):
+        sim = pbf_amd.Simulation(s.params, s.options)
+        sim.set_state(s.x, s.v)
+        sim.run(steps)
+        one = sim.get_state()
+        sim.close()
+        a, b = int(off[k]), int(off[k + 1])
+        assert np.array_equal(got["x"][a:b].view(np.uint32), one["x"].view(np.uint32)), k
+        assert np.array_equal(got["v"][a:b].view(np.uint32), one["v"].view(np.uint32)), k

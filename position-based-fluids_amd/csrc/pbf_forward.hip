@@ -6,21 +6,24 @@
 //   (onesweep passes, pbf_sort.hip)
 //   k_gather         sortParticles; recomputes x* from the gathered x, v
 //   k_cells          updateCells
-//   k_friends        buildFriendsList, 27-cell neighbourhood staged in LDS
+//   k_friends_direct buildFriendsList, candidates read from global memory
+//   / k_friends      (or from the workgroup's neighbourhood staged in LDS)
 //   K x { k_lambda   computeScaling + packData(lambda)
-//         k_delta    computeDelta (+ walls) + updatePredicted; on the last
-//                    iteration also packData(density) + updateVelocities }
+//         k_delta    computeDelta (+ walls and collision quads) +
+//                    updatePredicted; on the last iteration also
+//                    packData(density) + updateVelocities }
 //   k_viscosity      applyViscosity (Jacobi: reads a snapshot of v)
 //   k_vorticity      applyVorticity
 //
 // Every per-particle kernel runs over a block table (one {scene, first, end}
 // per workgroup), so one launch covers all scenes of a batch and the scene
 // constants of a workgroup are uniform (scalar loads).  Particles live in
-// cell-sorted order as float4 arrays; the friends list keeps the reference
-// layout (counts packed 5 x 6 bits per particle, indices[5][50][n]) so a wave
-// reads one friend slot of 64 consecutive particles as one 256-byte segment.
-// The sweeps walk the friends of a particle four at a time: four index loads,
-// then four gathers, then the four updates in the reference's order.
+// cell-sorted order as float4 arrays.  The friends build writes each
+// particle's circle lists (counts packed 5 x 6 bits) and the list a sweep
+// processes, grouped per 64 consecutive particles in 16-byte quads, so a
+// wave's load of one quad is a contiguous 1 KB (pbf_device.h).  The sweeps
+// walk a particle's friends a quad at a time: four gathers in flight, then
+// the four updates in the reference's order.
 #include "pbf_internal.h"
 
 namespace pbf {

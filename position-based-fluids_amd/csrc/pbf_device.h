@@ -152,6 +152,19 @@ __device__ __forceinline__ PairLen pair_len(float r2) {
 }
 __device__ __forceinline__ float norm_of_sq(float s2) { return s2 >= kTinySq ? sqrt_normal(s2) : 0.0f; }
 
+// Selection of a pair's contributions as arithmetic: lane_mask(keep) is all
+// ones when keep and zero otherwise, as a value the compiler cannot see
+// through (an empty asm statement), and masked(x, m) is x or +0 (also for a
+// NaN x).  The pair arithmetic then runs for every pair with no per-pair
+// branch around it: a branch that every lane of a wave takes anyway costs
+// exec-mask updates, zeroed phi registers and reloads of the scene constants.
+__device__ __forceinline__ uint32_t lane_mask(bool keep) {
+    uint32_t m = keep ? 0xFFFFFFFFu : 0u;
+    asm("" : "+v"(m));
+    return m;
+}
+__device__ __forceinline__ float masked(float x, uint32_t m) { return __uint_as_float(__float_as_uint(x) & m); }
+
 __device__ __forceinline__ int circle_count(uint32_t packed, int q) { return (packed >> (6 * q)) & 63; }
 
 // Number of friend circles a sweep processes.  The reference skips circle q
@@ -272,7 +285,11 @@ __device__ __forceinline__ void walk_quads(uint32_t i, int m, const uint4* __res
 // The same walk one quad deeper: the gathers of quad q + 1 are issued before
 // quad q is folded in, so a wave keeps its next gathers in flight while it
 // computes.  Entries past m are replaced by i, so every gather stays in
-// bounds; only the first m are added, in list order.
+// bounds.  add(g, valid) is called for all four entries of every quad, in
+// list order, with valid false for the entries past m: the accumulators fold
+// those in as +0, so the loop body has no per-entry branch.  The loop is
+// unrolled by two so that the two quads in flight keep their registers (no
+// copies between iterations).
 template <class Load, class Add>
 __device__ __forceinline__ void walk_quads_ahead(uint32_t i, int m, const uint4* __restrict__ quads, Load&& load,
                                                  Add&& add) {
@@ -283,27 +300,36 @@ __device__ __forceinline__ void walk_quads_ahead(uint32_t i, int m, const uint4*
         return make_uint4(b < m ? u.x : i, b + 1 < m ? u.y : i, b + 2 < m ? u.z : i, b + 3 < m ? u.w : i);
     };
     const uint4 first = own(quads[sweep_quad(i, 0)], 0);
-    auto g0 = load(first.x);
-    auto g1 = load(first.y);
-    auto g2 = load(first.z);
-    auto g3 = load(first.w);
+    auto a0 = load(first.x);
+    auto a1 = load(first.y);
+    auto a2 = load(first.z);
+    auto a3 = load(first.w);
     uint4 next = quads[sweep_quad(i, 1)];
-    for (int q = 0; q < nq; ++q) {
+    int q = 0;
+    for (;;) {
+        // quad q is in a0..a3, the indices of quad q + 1 in next
         const uint4 u = own(next, q + 1);
-        const auto h0 = load(u.x);
-        const auto h1 = load(u.y);
-        const auto h2 = load(u.z);
-        const auto h3 = load(u.w);
+        const auto b0 = load(u.x);
+        const auto b1 = load(u.y);
+        const auto b2 = load(u.z);
+        const auto b3 = load(u.w);
         next = quads[sweep_quad(i, min(q + 2, kQuads - 1))];   // a particle's 63 quads
-        const int b = 4 * q;
-        add(g0);
-        if (b + 1 < m) add(g1);
-        if (b + 2 < m) add(g2);
-        if (b + 3 < m) add(g3);
-        g0 = h0;
-        g1 = h1;
-        g2 = h2;
-        g3 = h3;
+        add(a0, true);
+        add(a1, 4 * q + 1 < m);
+        add(a2, 4 * q + 2 < m);
+        add(a3, 4 * q + 3 < m);
+        if (++q >= nq) break;
+        const uint4 w = own(next, q + 1);
+        a0 = load(w.x);
+        a1 = load(w.y);
+        a2 = load(w.z);
+        a3 = load(w.w);
+        next = quads[sweep_quad(i, min(q + 2, kQuads - 1))];
+        add(b0, true);
+        add(b1, 4 * q + 1 < m);
+        add(b2, 4 * q + 2 < m);
+        add(b3, 4 * q + 3 < m);
+        if (++q >= nq) break;
     }
 }
 
@@ -312,7 +338,7 @@ __device__ __forceinline__ void walk_friends(uint32_t i, uint32_t packed, const 
                                              Add&& add) {
     int m;
     (void)circles_processed<L_NUM, L_DEN>(packed, &m);
-    if (AHEAD)
+    if constexpr (AHEAD)
         walk_quads_ahead(i, m, quads, load, add);
     else
         walk_quads(i, m, quads, load, add);

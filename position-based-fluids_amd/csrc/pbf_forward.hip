@@ -538,20 +538,21 @@ struct Pair4 {
 // thread of this kernel reads another particle's w).
 struct LambdaAcc {
     float dens = 0.0f, gsum = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    // Branch-free: an out-of-range pair's contributions are selected to +0
-    // (no sum here is ever -0, so the result is bitwise that of skipping it,
-    // also when the friend's position is not finite) and no branch joins the
+    // Branch-free: the contributions of an out-of-range pair, and of a list
+    // entry past the walk's end (valid false), are selected to +0 (no sum
+    // here is ever -0, so the result is bitwise that of skipping it, also
+    // when the friend's position is not finite) and no branch joins the
     // loop, which keeps the next gathers in flight.
-    __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c) {
+    __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c, bool valid) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        const bool in = r2 < c.h2;
+        const uint32_t in = lane_mask(valid && r2 < c.h2);
         const PairLen L = pair_len(r2);   // r/|r| = r * (1/|r|); 0 for a coincident pair
         const float hr = c.h - L.rl;
         const float s = c.spiky * hr * hr;
         const float t = s * L.inv;
-        const float ax = in ? rx * t : 0.0f, ay = in ? ry * t : 0.0f, az = in ? rz * t : 0.0f;
-        const float d = in ? c.h2 - r2 : 0.0f;
+        const float ax = masked(rx * t, in), ay = masked(ry * t, in), az = masked(rz * t, in);
+        const float d = masked(c.h2 - r2, in);
         dens += d * d * d;
         gsum += ax * ax + ay * ay + az * az;
         gx += ax;
@@ -574,7 +575,8 @@ __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ s
     const float4 p = xs[i];
     LambdaAcc a;
     walk_friends<3, 5, true>(
-        i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; }, [&](const float4& q) { a.add(p, q, c); });
+        i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
+        [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
     const float dens = a.dens * c.poly6;
     rho[i] = dens;
     const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
@@ -590,10 +592,10 @@ __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ s
 // pack_data.cl (density into x*.w) and update_velocities.cl.
 struct DeltaAcc {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c) {
+    __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c, bool valid) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        const bool in = r2 < c.h2;
+        const uint32_t in = lane_mask(valid && r2 < c.h2);   // selected to +0 otherwise (as in LambdaAcc)
         const PairLen L = pair_len(r2);
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
@@ -603,9 +605,9 @@ struct DeltaAcc {
         const float ratio = p6 * c.inv_poly6_q;
         const float scorr = c.st_k * ratio * ratio * ratio * ratio;
         const float coef = p.w + pj.w + scorr;
-        sx += in ? coef * ax : 0.0f;
-        sy += in ? coef * ay : 0.0f;
-        sz += in ? coef * az : 0.0f;
+        sx += masked(coef * ax, in);
+        sy += masked(coef * ay, in);
+        sz += masked(coef * az, in);
     }
 };
 
@@ -624,7 +626,8 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
     const float4 p = xs[i];
     DeltaAcc a;
     walk_friends<1, 2, true>(
-        i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; }, [&](const float4& q) { a.add(p, q, c); });
+        i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
+        [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
     float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
     float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
     float fz = p.z + (-c.spiky * a.sz) / c.rest_density;
@@ -657,24 +660,24 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
 struct ViscAcc {
     float vsx = 0.0f, vsy = 0.0f, vsz = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
     __device__ __forceinline__ void add(const float4& p, const float4& vi, const float4& pj, const float4& vj,
-                                        const SceneConsts& c) {
+                                        const SceneConsts& c, bool valid) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         // a skipped pair's contributions are selected to +0 (as in LambdaAcc)
-        const bool in = r2 < c.h2 && fabsf(pj.w) > 1e-8f;
+        const uint32_t in = lane_mask(valid && r2 < c.h2 && fabsf(pj.w) > 1e-8f);
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float d = c.h2 - r2;
         const PairLen L = pair_len(r2);
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
-        ox += in ? uy * az - uz * ay : 0.0f;
-        oy += in ? uz * ax - ux * az : 0.0f;
-        oz += in ? ux * ay - uy * ax : 0.0f;
+        ox += masked(uy * az - uz * ay, in);
+        oy += masked(uz * ax - ux * az, in);
+        oz += masked(ux * ay - uy * ax, in);
         const float p6 = d * d * d;
-        vsx += in ? ux * p6 : 0.0f;
-        vsy += in ? uy * p6 : 0.0f;
-        vsz += in ? uz * p6 : 0.0f;
+        vsx += masked(ux * p6, in);
+        vsy += masked(uy * p6, in);
+        vsz += masked(uz * p6, in);
     }
 };
 
@@ -693,7 +696,7 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
     ViscAcc a;
     walk_friends<1, 2, true>(
         i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], v_in[j]}; },
-        [&](const Pair4& t) { a.add(p, vi, t.a, t.b, c); });
+        [&](const Pair4& t, bool valid) { a.add(p, vi, t.a, t.b, c, valid); });
     float4 vo;
     vo.x = vi.x + c.viscosity * (a.vsx * c.poly6);
     vo.y = vi.y + c.viscosity * (a.vsy * c.poly6);
@@ -709,15 +712,15 @@ struct VortAcc {
     __device__ __forceinline__ void add(const float4& p, const float4& pj, const float4& oj, const SceneConsts& c) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        const bool in = r2 < c.h2 && fabsf(pj.w) > 1e-8f;
+        const uint32_t in = lane_mask(r2 < c.h2 && fabsf(pj.w) > 1e-8f);
         const PairLen L = pair_len(r2);
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
         const float wl = norm_of_sq(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
-        ex += in ? wl * ax : 0.0f;
-        ey += in ? wl * ay : 0.0f;
-        ez += in ? wl * az : 0.0f;
+        ex += masked(wl * ax, in);
+        ey += masked(wl * ay, in);
+        ez += masked(wl * az, in);
     }
 };
 

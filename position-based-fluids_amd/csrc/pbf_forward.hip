@@ -200,8 +200,9 @@ __device__ __forceinline__ int wave_max(int v) {
 __device__ __forceinline__ int friend_circle(const float4& p, const float4& q, const SceneConsts& c) {
     const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    if (!(r2 < c.h2)) return -1;
-    return (int)(r2 >= c.circ_t[0]) + (int)(r2 >= c.circ_t[1]) + (int)(r2 >= c.circ_t[2]) + (int)(r2 >= c.circ_t[3]);
+    const int circ =
+        (int)(r2 >= c.circ_t[0]) + (int)(r2 >= c.circ_t[1]) + (int)(r2 >= c.circ_t[2]) + (int)(r2 >= c.circ_t[3]);
+    return (int)((uint32_t)circ | ~lane_mask(r2 < c.h2));   // -1 out of range, computed without a branch
 }
 
 // Appends friend j of circle `circ` to particle i's list under the 50 cap.
@@ -234,8 +235,13 @@ struct FriendWriter {
         uint32_t* col = stage + 4 * circ * BS;
         col[(f & 3u) * BS] = j;
         if ((f & 3u) == 3u) {
-            const uint32_t qo = (circ == 0 ? (f >> 2) : (uint32_t)(circ - 1) * kSegQuads + (f >> 2)) * 64u;
-            (circ == 0 ? qbase0 : qbase1)[qo] = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
+            // quad f / 4 of the circle's segment, addressed without a branch on
+            // the circle (circle 0 in the sweep quads, the others in the circle
+            // quads)
+            const uint32_t c0 = lane_mask(circ == 0);
+            const uint32_t qo = (((uint32_t)(circ - 1) * kSegQuads) & ~c0) + (f >> 2);
+            uint4* base = c0 ? qbase0 : qbase1;
+            base[(size_t)qo * 64u] = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
         }
         packed += 1u << (6 * circ);
     }

@@ -71,8 +71,20 @@ def family(name: str) -> str:
     return tail[0] if len(tail) == 2 and tail[1].isdigit() else name
 
 
-def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
-    sys.path.insert(0, str(ROOT / "oracle"))
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_rate(scene, max_steps: int, budget_s: float) -> tuple[float, int, float]:
+    """particle-substeps/s of the CPU oracle from the scene's initial state
+    (one untimed warm-up substep, then up to max_steps or budget_s)."""
     import pbf_oracle
     o = pbf_oracle.OracleSim(pbf_oracle.cfg_from_params(scene.params, scene.options))
     o.set_state(scene.x, scene.v)
@@ -83,16 +95,30 @@ def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
         o.step()
         steps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 20:
+        if el >= budget_s or steps >= max_steps:
             break
     o.close()
+    return scene.n * steps / el, steps, el
+
+
+def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pbf_oracle
+    import pbf_amd
+    rate, steps, el = _oracle_rate(scene, 20, budget_s)
+    # the 16,384-particle dam break over its full 200 substeps
+    small = pbf_amd.config("dam16k", seed=0)
+    rate16k, steps16k, el16k = _oracle_rate(small, 200, budget_s)
     return {
-        "value": scene.n * steps / el,
+        "value": rate,
         "unit": "particle-substeps/s",
         "cores": int(pbf_oracle.lib().pbfo_threads()),
         "kind": "port",
         "sample": f"dam1m substeps 2..{steps + 1} from the initial state ({steps} timed, {el:.1f} s), "
                   f"C/OpenMP restatement of the reference kernels",
+        "host": {"nproc": os.cpu_count(), "model": _cpu_model()},
+        "dam16k": {"value": rate16k, "steps": steps16k, "seconds": round(el16k, 2),
+                   "ms_per_substep": round(el16k / steps16k * 1e3, 3)},
     }
 
 

@@ -715,10 +715,11 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
 // apply_vorticity.cl:1-87
 struct VortAcc {
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    __device__ __forceinline__ void add(const float4& p, const float4& pj, const float4& oj, const SceneConsts& c) {
+    __device__ __forceinline__ void add(const float4& p, const float4& pj, const float4& oj, const SceneConsts& c,
+                                        bool valid) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
-        const uint32_t in = lane_mask(r2 < c.h2 && fabsf(pj.w) > 1e-8f);
+        const uint32_t in = lane_mask(valid && r2 < c.h2 && fabsf(pj.w) > 1e-8f);
         const PairLen L = pair_len(r2);
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
@@ -742,9 +743,9 @@ __global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict_
     if (i >= be.z) return;
     const float4 p = xs[i];
     VortAcc a;
-    walk_friends<1, 2>(
+    walk_friends<1, 2, true>(
         i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], omega[j]}; },
-        [&](const Pair4& t) { a.add(p, t.a, t.b, c); });
+        [&](const Pair4& t, bool valid) { a.add(p, t.a, t.b, c, valid); });
     float ex = a.ex * -c.spiky, ey = a.ey * -c.spiky, ez = a.ez * -c.spiky;
     const float l = sqrtf(ex * ex + ey * ey + ez * ez);
     if (l > 0.0f) { ex /= l; ey /= l; ez /= l; }

@@ -185,10 +185,19 @@ int pbf_set_workgroup_sizes(pbf_ctx *ctx, const uint32_t *sizes /* PBF_FAMILIES 
 /* Friends build variant: 1 stages each tile's 27-cell neighbourhood in LDS,
  * 0 reads candidates from global memory (L2).  Both give the same lists. */
 int pbf_set_friends_mode(pbf_ctx *ctx, int staged);
+/* Tiles per workgroup (the particles each thread handles in turn) of every
+ * family: 1 launches one workgroup per tile of the block table; k > 1
+ * launches about tiles / k workgroups; 0 launches a persistent grid, as many
+ * workgroups as the device holds at once for the family's kernel and size.
+ * Workgroups of a strided launch walk their XCD's run of tiles.  The
+ * LDS-staged friends build always runs one tile per workgroup.  Results do
+ * not depend on it.  Values up to 64. */
+int pbf_set_tiles_per_workgroup(pbf_ctx *ctx, const uint32_t *tiles /* PBF_FAMILIES */);
 /* Autotuner: times every candidate size for each family (and both friends
- * build variants) over `steps`
- * substeps from the current state (which is restored afterwards), keeps the
- * fastest, and returns the chosen sizes in `chosen` (may be NULL). */
+ * build variants), then 4 tiles per workgroup and the persistent grid at the
+ * chosen sizes, over `steps` substeps from the current state (which is
+ * restored afterwards), keeps the fastest, and returns the chosen sizes in
+ * `chosen` (may be NULL; pbf_stats reports the tiles per workgroup). */
 int pbf_autotune(pbf_ctx *ctx, uint32_t steps, uint32_t *chosen /* PBF_FAMILIES */);
 
 typedef struct pbf_stats_t {
@@ -197,7 +206,8 @@ typedef struct pbf_stats_t {
     uint32_t unstaged_tiles;      /* friends tiles that read global memory */
     uint32_t workgroup_size[PBF_FAMILIES];
     uint32_t friends_staged;      /* friends build variant (pbf_set_friends_mode) */
-    uint32_t reserved[7];
+    uint32_t tiles_per_workgroup[PBF_FAMILIES]; /* pbf_set_tiles_per_workgroup */
+    uint32_t reserved[2];
 } pbf_stats_t;
 int pbf_stats(pbf_ctx *ctx, pbf_stats_t *out);
 

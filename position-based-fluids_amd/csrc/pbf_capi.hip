@@ -231,6 +231,8 @@ int create_impl(const pbf_params* params, const pbf_options* opts, uint32_t S, p
     } while (0)
 
     CREATE_TRY(hipGetDevice(&ctx->device));
+    CREATE_TRY(hipDeviceGetAttribute(&ctx->cu_count, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    CREATE_TRY(pbf::query_residency(ctx));
     CREATE_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     const size_t n = ctx->n;
     CREATE_TRY(dalloc(&ctx->x, n));
@@ -805,7 +807,10 @@ int pbf_stats(pbf_ctx* ctx, pbf_stats_t* out) {
     out->total_cells = ctx->total_cells;
     out->particles = ctx->n;
     out->scenes = ctx->S;
-    for (int f = 0; f < pbf::F_COUNT; ++f) out->workgroup_size[f] = 1u << ctx->bs_log2[f];
+    for (int f = 0; f < pbf::F_COUNT; ++f) {
+        out->workgroup_size[f] = 1u << ctx->bs_log2[f];
+        out->tiles_per_workgroup[f] = ctx->tiles_per_wg[f];
+    }
     return PBF_OK;
 }
 
@@ -829,6 +834,16 @@ int pbf_set_workgroup_sizes(pbf_ctx* ctx, const uint32_t* sizes) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     drop_graphs(ctx);
     for (int f = 0; f < pbf::F_COUNT; ++f) ctx->bs_log2[f] = lg[f];
+    return PBF_OK;
+}
+
+int pbf_set_tiles_per_workgroup(pbf_ctx* ctx, const uint32_t* tiles) {
+    if (!ctx || !tiles) return PBF_E_INVALID;
+    for (int f = 0; f < pbf::F_COUNT; ++f)
+        if (tiles[f] > 64) return fail(ctx, PBF_E_INVALID, "tiles per workgroup %u > 64", tiles[f]);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    for (int f = 0; f < pbf::F_COUNT; ++f) ctx->tiles_per_wg[f] = tiles[f];
     return PBF_OK;
 }
 
@@ -871,43 +886,72 @@ int pbf_autotune(pbf_ctx* ctx, uint32_t steps, uint32_t* chosen) {
         hipMemcpyAsync(sv, ctx->v, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
         rc = fail(ctx, PBF_E_HIP, "pbf_autotune: state copy failed");
     }
+    // one timed trial from the saved state: per-family milliseconds in ms
+    auto trial_ms = [&](double* ms) -> int {
+        int r;
+        if ((r = restore()) != PBF_OK) return r;
+        ctx->profiling = false;
+        if ((r = enqueue_step(ctx, 0)) != PBF_OK) return r;   // warm-up
+        ctx->trackers.clear();
+        ctx->profiling = true;
+        for (uint32_t s = 0; s < steps && r == PBF_OK; ++s) r = enqueue_step(ctx, 0);
+        if (r != PBF_OK) return r;
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, PBF_E_HIP, "pbf_autotune: sync");
+        pbf::collect_timings(ctx);
+        ctx->profiling = false;
+        for (int f = 0; f < pbf::F_COUNT; ++f) {
+            ms[f] = 0.0;
+            const size_t len = strlen(kFamilyTrackers[f]);
+            for (const pbf::Tracker& t : ctx->trackers)
+                if (t.name.compare(0, len, kFamilyTrackers[f]) == 0) ms[f] += t.t.sum_ms;
+        }
+        return PBF_OK;
+    };
+    // workgroup sizes (and the friends build variant), one tile per workgroup
     double best_mode_ms[2] = {1e300, 1e300};
     int best_mode_l[2] = {8, 8};
+    for (int f = 0; f < pbf::F_COUNT; ++f) ctx->tiles_per_wg[f] = 1;
     for (int trial = 0; trial < 8 && rc == PBF_OK; ++trial) {
         const int l = 6 + (trial & 3);
         const bool staged_trial = trial >= 4;
         ctx->friends_staged = staged_trial;
         for (int f = 0; f < pbf::F_COUNT; ++f) ctx->bs_log2[f] = l;
-        if ((rc = restore()) != PBF_OK) break;
-        ctx->profiling = false;
-        if ((rc = enqueue_step(ctx, 0)) != PBF_OK) break;   // warm-up
-        ctx->trackers.clear();
-        ctx->profiling = true;
-        for (uint32_t s = 0; s < steps && rc == PBF_OK; ++s) rc = enqueue_step(ctx, 0);
-        if (rc != PBF_OK) break;
-        if (hipStreamSynchronize(ctx->stream) != hipSuccess) { rc = fail(ctx, PBF_E_HIP, "pbf_autotune: sync"); break; }
-        pbf::collect_timings(ctx);
-        ctx->profiling = false;
+        double ms[pbf::F_COUNT];
+        if ((rc = trial_ms(ms)) != PBF_OK) break;
         for (int f = 0; f < pbf::F_COUNT; ++f) {
-            double ms = 0.0;
-            const size_t len = strlen(kFamilyTrackers[f]);
-            for (const pbf::Tracker& t : ctx->trackers)
-                if (t.name.compare(0, len, kFamilyTrackers[f]) == 0) ms += t.t.sum_ms;
             if (f == pbf::F_FRIENDS) {
-                if (ms < best_mode_ms[staged_trial]) {
-                    best_mode_ms[staged_trial] = ms;
+                if (ms[f] < best_mode_ms[staged_trial]) {
+                    best_mode_ms[staged_trial] = ms[f];
                     best_mode_l[staged_trial] = l;
                 }
-            } else if (!staged_trial && ms < best[f]) {   // the other families run in every trial
-                best[f] = ms;
+            } else if (!staged_trial && ms[f] < best[f]) {   // the other families run in every trial
+                best[f] = ms[f];
                 best_l[f] = l;
             }
         }
     }
     ctx->friends_staged = best_mode_ms[1] < best_mode_ms[0];
     best_l[pbf::F_FRIENDS] = best_mode_l[ctx->friends_staged ? 1 : 0];
+    best[pbf::F_FRIENDS] = best_mode_ms[ctx->friends_staged ? 1 : 0];
+    for (int f = 0; f < pbf::F_COUNT; ++f) ctx->bs_log2[f] = best_l[f];
+    // tiles per workgroup at those sizes: 4, then a persistent grid, against
+    // one tile per workgroup (the staged friends build runs one tile each)
+    uint32_t best_t[pbf::F_COUNT] = {1, 1, 1, 1, 1};
+    static const uint32_t kTileTrials[2] = {4, 0};
+    for (int k = 0; k < 2 && rc == PBF_OK; ++k) {
+        for (int f = 0; f < pbf::F_COUNT; ++f) ctx->tiles_per_wg[f] = kTileTrials[k];
+        double ms[pbf::F_COUNT];
+        if ((rc = trial_ms(ms)) != PBF_OK) break;
+        for (int f = 0; f < pbf::F_COUNT; ++f) {
+            if (f == pbf::F_FRIENDS && ctx->friends_staged) continue;
+            if (ms[f] < best[f]) {
+                best[f] = ms[f];
+                best_t[f] = kTileTrials[k];
+            }
+        }
+    }
     for (int f = 0; f < pbf::F_COUNT; ++f) {
-        ctx->bs_log2[f] = best_l[f];
+        ctx->tiles_per_wg[f] = best_t[f];
         if (chosen) chosen[f] = 1u << best_l[f];
     }
     if (rc == PBF_OK) rc = restore();

@@ -24,6 +24,8 @@
 // wave's load of one quad is a contiguous 1 KB (pbf_device.h).  The sweeps
 // walk a particle's friends a quad at a time: four gathers in flight, then
 // the four updates in the reference's order.
+#include <algorithm>
+
 #include "pbf_internal.h"
 
 namespace pbf {
@@ -369,27 +371,30 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
                                                       const uint4* __restrict__ blocks, const float4* __restrict__ xs,
                                                       const uint2* __restrict__ cells, uint32_t* __restrict__ nb_cnt,
                                                       uint32_t* __restrict__ nb_list, uint4* __restrict__ quads,
-                                                      uint32_t n, uint32_t fresh, uint32_t* err) {
+                                                      uint32_t n, uint32_t fresh, uint32_t* err,
+                                                      uint32_t ntiles) {
     __shared__ uint32_t s_stage[4 * kCircles * BS];
-    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
-    const SceneConsts& c = scenes[be.x];
-    const uint32_t i = be.y + threadIdx.x;
-    if (i >= be.z) return;
-    uint4* list = reinterpret_cast<uint4*>(nb_list);
-    const float4 p = xs[i];
-    const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
-    FriendWriter<BS> w;
-    w.stage = s_stage + threadIdx.x;
-    w.bind(i, list, quads);
-    const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dz = -1; dz <= 1; ++dz)
-                if (scan & (1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1))))
-                    scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list, quads,
-                                       w);
-    w.finish(i, nb_cnt, list, quads, err);
-    pack_friends(i, w.packed, list, quads);
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+        const uint4 be = blocks[xcd_tile(tv, ntiles)];
+        const SceneConsts& c = scenes[be.x];
+        const uint32_t i = be.y + threadIdx.x;
+        if (i >= be.z) continue;
+        uint4* list = reinterpret_cast<uint4*>(nb_list);
+        const float4 p = xs[i];
+        const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
+        FriendWriter<BS> w;
+        w.stage = s_stage + threadIdx.x;
+        w.bind(i, list, quads);
+        const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz)
+                    if (scan & (1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1))))
+                        scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list,
+                                           quads, w);
+        w.finish(i, nb_cnt, list, quads, err);
+        pack_friends(i, w.packed, list, quads);
+    }
 }
 
 // Staging budget of the LDS variant per workgroup size: box cells, particles.
@@ -573,25 +578,28 @@ template <int BS, bool TAPE>
 __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
                                               float4* __restrict__ xs, const uint32_t* __restrict__ nb_cnt,
                                               const uint4* __restrict__ nb_list, float* __restrict__ rho,
-                                              float* __restrict__ lambda, float4* __restrict__ aux, uint32_t n) {
-    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
-    const SceneConsts& c = scenes[be.x];
-    const uint32_t i = be.y + threadIdx.x;
-    if (i >= be.z) return;
-    const float4 p = xs[i];
-    LambdaAcc a;
-    walk_friends<3, 5, true>(
-        i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
-        [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
-    const float dens = a.dens * c.poly6;
-    rho[i] = dens;
-    const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
-    const float C = (dens / c.rest_density) - 1.0f;
-    const float den = gsum / (c.rest_density * c.rest_density) + c.epsilon;
-    const float lam = -1.0f * C / den;
-    lambda[i] = lam;
-    reinterpret_cast<float*>(xs + i)[3] = lam;
-    if (TAPE) aux[i] = make_float4(a.gx, a.gy, a.gz, den);
+                                              float* __restrict__ lambda, float4* __restrict__ aux, uint32_t n,
+                                              uint32_t ntiles) {
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+        const uint4 be = blocks[xcd_tile(tv, ntiles)];
+        const SceneConsts& c = scenes[be.x];
+        const uint32_t i = be.y + threadIdx.x;
+        if (i >= be.z) continue;
+        const float4 p = xs[i];
+        LambdaAcc a;
+        walk_friends<3, 5, true>(
+            i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
+            [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
+        const float dens = a.dens * c.poly6;
+        rho[i] = dens;
+        const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
+        const float C = (dens / c.rest_density) - 1.0f;
+        const float den = gsum / (c.rest_density * c.rest_density) + c.epsilon;
+        const float lam = -1.0f * C / den;
+        lambda[i] = lam;
+        reinterpret_cast<float*>(xs + i)[3] = lam;
+        if (TAPE) aux[i] = make_float4(a.gx, a.gy, a.gz, den);
+    }
 }
 
 // compute_delta.cl:44-186 + update_predicted.cl; on the last iteration also
@@ -624,41 +632,44 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
                                              const float4* __restrict__ xs, float4* __restrict__ xs_next,
                                              float4* __restrict__ x, float4* __restrict__ v,
                                              const float* __restrict__ rho, const uint32_t* __restrict__ nb_cnt,
-                                             const uint4* __restrict__ nb_list, float4* __restrict__ fut, uint32_t n) {
-    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
-    const SceneConsts& c = scenes[be.x];
-    const uint32_t i = be.y + threadIdx.x;
-    if (i >= be.z) return;
-    const float4 p = xs[i];
-    DeltaAcc a;
-    walk_friends<1, 2, true>(
-        i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
-        [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
-    float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
-    float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
-    float fz = p.z + (-c.spiky * a.sz) / c.rest_density;
-    const float4 prev = x[i];
-    const float ux = fx, uy = fy, uz = fz;
-    uint32_t hits = 0;
-    if (c.walls | c.quad_count) hits = apply_walls(fx, fy, fz, prev, c);
-    if (TAPE) fut[i] = make_float4(ux, uy, uz, __uint_as_float(hits));
-    const float dx = fx - p.x, dy = fy - p.y, dz = fz - p.z;
-    float4 o;
-    o.x = p.x + dx;
-    o.y = p.y + dy;
-    o.z = p.z + dz;
-    if (!LAST) {
-        o.w = p.w;
-        xs_next[i] = o;
-    } else {
-        o.w = rho[i];
-        xs_next[i] = o;
-        const float vx = (o.x - prev.x) / c.dt, vy = (o.y - prev.y) / c.dt, vz = (o.z - prev.z) / c.dt;
-        float* vp = reinterpret_cast<float*>(v + i);
-        vp[0] = vx;
-        vp[1] = vy;
-        vp[2] = vz;
-        x[i] = make_float4(o.x, o.y, o.z, sqrtf(vx * vx + vy * vy + vz * vz));
+                                             const uint4* __restrict__ nb_list, float4* __restrict__ fut, uint32_t n,
+                                             uint32_t ntiles) {
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+        const uint4 be = blocks[xcd_tile(tv, ntiles)];
+        const SceneConsts& c = scenes[be.x];
+        const uint32_t i = be.y + threadIdx.x;
+        if (i >= be.z) continue;
+        const float4 p = xs[i];
+        DeltaAcc a;
+        walk_friends<1, 2, true>(
+            i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
+            [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
+        float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
+        float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
+        float fz = p.z + (-c.spiky * a.sz) / c.rest_density;
+        const float4 prev = x[i];
+        const float ux = fx, uy = fy, uz = fz;
+        uint32_t hits = 0;
+        if (c.walls | c.quad_count) hits = apply_walls(fx, fy, fz, prev, c);
+        if (TAPE) fut[i] = make_float4(ux, uy, uz, __uint_as_float(hits));
+        const float dx = fx - p.x, dy = fy - p.y, dz = fz - p.z;
+        float4 o;
+        o.x = p.x + dx;
+        o.y = p.y + dy;
+        o.z = p.z + dz;
+        if (!LAST) {
+            o.w = p.w;
+            xs_next[i] = o;
+        } else {
+            o.w = rho[i];
+            xs_next[i] = o;
+            const float vx = (o.x - prev.x) / c.dt, vy = (o.y - prev.y) / c.dt, vz = (o.z - prev.z) / c.dt;
+            float* vp = reinterpret_cast<float*>(v + i);
+            vp[0] = vx;
+            vp[1] = vy;
+            vp[2] = vz;
+            x[i] = make_float4(o.x, o.y, o.z, sqrtf(vx * vx + vy * vy + vz * vz));
+        }
     }
 }
 
@@ -692,24 +703,26 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
                                                  const uint4* __restrict__ blocks, const float4* __restrict__ xs,
                                                  const float4* __restrict__ v_in, float4* __restrict__ v_out,
                                                  float4* __restrict__ omega, const uint32_t* __restrict__ nb_cnt,
-                                                 const uint4* __restrict__ nb_list, uint32_t n) {
-    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
-    const SceneConsts& c = scenes[be.x];
-    const uint32_t i = be.y + threadIdx.x;
-    if (i >= be.z) return;
-    const float4 p = xs[i];
-    const float4 vi = v_in[i];
-    ViscAcc a;
-    walk_friends<1, 2, true>(
-        i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], v_in[j]}; },
-        [&](const Pair4& t, bool valid) { a.add(p, vi, t.a, t.b, c, valid); });
-    float4 vo;
-    vo.x = vi.x + c.viscosity * (a.vsx * c.poly6);
-    vo.y = vi.y + c.viscosity * (a.vsy * c.poly6);
-    vo.z = vi.z + c.viscosity * (a.vsz * c.poly6);
-    vo.w = vi.w + c.viscosity * 0.0f;
-    v_out[i] = vo;
-    omega[i] = make_float4(a.ox * -c.spiky, a.oy * -c.spiky, a.oz * -c.spiky, 0.0f);
+                                                 const uint4* __restrict__ nb_list, uint32_t n, uint32_t ntiles) {
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+        const uint4 be = blocks[xcd_tile(tv, ntiles)];
+        const SceneConsts& c = scenes[be.x];
+        const uint32_t i = be.y + threadIdx.x;
+        if (i >= be.z) continue;
+        const float4 p = xs[i];
+        const float4 vi = v_in[i];
+        ViscAcc a;
+        walk_friends<1, 2, true>(
+            i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], v_in[j]}; },
+            [&](const Pair4& t, bool valid) { a.add(p, vi, t.a, t.b, c, valid); });
+        float4 vo;
+        vo.x = vi.x + c.viscosity * (a.vsx * c.poly6);
+        vo.y = vi.y + c.viscosity * (a.vsy * c.poly6);
+        vo.z = vi.z + c.viscosity * (a.vsz * c.poly6);
+        vo.w = vi.w + c.viscosity * 0.0f;
+        v_out[i] = vo;
+        omega[i] = make_float4(a.ox * -c.spiky, a.oy * -c.spiky, a.oz * -c.spiky, 0.0f);
+    }
 }
 
 // apply_vorticity.cl:1-87
@@ -736,28 +749,30 @@ __global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict_
                                                  const uint4* __restrict__ blocks, const float4* __restrict__ xs,
                                                  float4* __restrict__ v, const float4* __restrict__ omega,
                                                  const uint32_t* __restrict__ nb_cnt,
-                                                 const uint4* __restrict__ nb_list, uint32_t n) {
-    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
-    const SceneConsts& c = scenes[be.x];
-    const uint32_t i = be.y + threadIdx.x;
-    if (i >= be.z) return;
-    const float4 p = xs[i];
-    VortAcc a;
-    walk_friends<1, 2, true>(
-        i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], omega[j]}; },
-        [&](const Pair4& t, bool valid) { a.add(p, t.a, t.b, c, valid); });
-    float ex = a.ex * -c.spiky, ey = a.ey * -c.spiky, ez = a.ez * -c.spiky;
-    const float l = sqrtf(ex * ex + ey * ey + ez * ez);
-    if (l > 0.0f) { ex /= l; ey /= l; ez /= l; }
-    const float4 oi = omega[i];
-    const float cx = ey * oi.z - ez * oi.y;
-    const float cy = ez * oi.x - ex * oi.z;
-    const float cz = ex * oi.y - ey * oi.x;
-    float4 vi = v[i];
-    vi.x += (c.vorticity * cx) * c.dt;
-    vi.y += (c.vorticity * cy) * c.dt;
-    vi.z += (c.vorticity * cz) * c.dt;
-    v[i] = vi;
+                                                 const uint4* __restrict__ nb_list, uint32_t n, uint32_t ntiles) {
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+        const uint4 be = blocks[xcd_tile(tv, ntiles)];
+        const SceneConsts& c = scenes[be.x];
+        const uint32_t i = be.y + threadIdx.x;
+        if (i >= be.z) continue;
+        const float4 p = xs[i];
+        VortAcc a;
+        walk_friends<1, 2, true>(
+            i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], omega[j]}; },
+            [&](const Pair4& t, bool valid) { a.add(p, t.a, t.b, c, valid); });
+        float ex = a.ex * -c.spiky, ey = a.ey * -c.spiky, ez = a.ez * -c.spiky;
+        const float l = sqrtf(ex * ex + ey * ey + ez * ez);
+        if (l > 0.0f) { ex /= l; ey /= l; ez /= l; }
+        const float4 oi = omega[i];
+        const float cx = ey * oi.z - ez * oi.y;
+        const float cy = ez * oi.x - ex * oi.z;
+        const float cz = ex * oi.y - ey * oi.x;
+        float4 vi = v[i];
+        vi.x += (c.vorticity * cx) * c.dt;
+        vi.y += (c.vorticity * cy) * c.dt;
+        vi.z += (c.vorticity * cz) * c.dt;
+        v[i] = vi;
+    }
 }
 
 }  // namespace
@@ -770,6 +785,39 @@ __global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict_
         case 9: { constexpr int BS = 512; LAUNCH; } break;   \
         default: { constexpr int BS = 256; LAUNCH; } break;  \
     }
+
+// Workgroups of a launch over `tiles` tiles for family f: one per tile, or,
+// when the family runs k > 1 tiles per workgroup (or persistently, k = 0),
+// fewer: a multiple of the XCD count, so that the dispatch indices a
+// workgroup strides over (b, b + grid, ...) stay on its XCD's run of tiles
+// (xcd_tile).
+uint32_t launch_grid(const pbf_ctx* ctx, Family f, uint32_t tiles) {
+    const uint32_t k = ctx->tiles_per_wg[f];
+    if (k == 1 || tiles <= kXcds) return tiles;
+    uint32_t g = k == 0 ? ctx->resident[f][ctx->bs_log2[f] - 6] : (tiles + k - 1) / k;
+    g = std::max<uint32_t>(kXcds, g / kXcds * kXcds);
+    return std::min(g, tiles);
+}
+
+template <class K>
+uint32_t resident_blocks(const pbf_ctx* ctx, K kernel, int bs) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return (uint32_t)per_cu * (uint32_t)std::max(ctx->cu_count, 1);
+}
+
+hipError_t query_residency(pbf_ctx* ctx) {
+    for (int l = 6; l <= 9; ++l) {
+        PBF_DISPATCH_BS(l, {
+            ctx->resident[F_FRIENDS][l - 6] = resident_blocks(ctx, k_friends_direct<BS>, BS);
+            ctx->resident[F_LAMBDA][l - 6] = resident_blocks(ctx, k_lambda<BS, false>, BS);
+            ctx->resident[F_DELTA][l - 6] = resident_blocks(ctx, k_delta<BS, false, false>, BS);
+            ctx->resident[F_VISC][l - 6] = resident_blocks(ctx, k_viscosity<BS>, BS);
+            ctx->resident[F_VORT][l - 6] = resident_blocks(ctx, k_vorticity<BS>, BS);
+        });
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_predict_hash(pbf_ctx* ctx, bool reset_old) {
     track_begin(ctx, "predictPositions");
@@ -823,9 +871,10 @@ hipError_t launch_friends(pbf_ctx* ctx) {
                                            ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, fresh, ctx->err));
     } else {
         PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS],
-                        hipLaunchKernelGGL(k_friends_direct<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
-                                           t.d, ctx->xs[ctx->xs_cur], reinterpret_cast<const uint2*>(ctx->cells),
-                                           ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, fresh, ctx->err));
+                        hipLaunchKernelGGL(k_friends_direct<BS>, dim3(launch_grid(ctx, F_FRIENDS, t.count)), dim3(BS),
+                                           0, ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur],
+                                           reinterpret_cast<const uint2*>(ctx->cells), ctx->nb_cnt, ctx->nb_list,
+                                           ctx->nb_quads, ctx->n, fresh, ctx->err, t.count));
     }
     track_end(ctx);
     return hipGetLastError();
@@ -836,14 +885,14 @@ hipError_t launch_lambda(pbf_ctx* ctx, int iter, float4* aux) {
     const BlockTable& t = table_for(ctx, F_LAMBDA);
     if (aux) {
         PBF_DISPATCH_BS(ctx->bs_log2[F_LAMBDA],
-                        hipLaunchKernelGGL((k_lambda<BS, true>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
-                                           t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt, ctx->nb_quads, ctx->rho,
-                                           ctx->lambda, aux, ctx->n));
+                        hipLaunchKernelGGL((k_lambda<BS, true>), dim3(launch_grid(ctx, F_LAMBDA, t.count)), dim3(BS),
+                                           0, ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt,
+                                           ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n, t.count));
     } else {
         PBF_DISPATCH_BS(ctx->bs_log2[F_LAMBDA],
-                        hipLaunchKernelGGL((k_lambda<BS, false>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d,
-                                           t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt, ctx->nb_quads, ctx->rho,
-                                           ctx->lambda, aux, ctx->n));
+                        hipLaunchKernelGGL((k_lambda<BS, false>), dim3(launch_grid(ctx, F_LAMBDA, t.count)),
+                                           dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur],
+                                           ctx->nb_cnt, ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n, t.count));
     }
     track_end(ctx);
     return hipGetLastError();
@@ -855,9 +904,10 @@ hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last, float4* fut) {
     const BlockTable& t = table_for(ctx, F_DELTA);
 #define PBF_LAUNCH_DELTA(LASTV, TAPEV)                                                                            \
     PBF_DISPATCH_BS(ctx->bs_log2[F_DELTA],                                                                       \
-                    hipLaunchKernelGGL((k_delta<BS, LASTV, TAPEV>), dim3(t.count), dim3(BS), 0, ctx->stream,      \
-                                       ctx->sc_d, t.d, ctx->xs[cur], ctx->xs[1 - cur], ctx->x, ctx->v, ctx->rho,  \
-                                       ctx->nb_cnt, ctx->nb_quads, fut, ctx->n))
+                    hipLaunchKernelGGL((k_delta<BS, LASTV, TAPEV>), dim3(launch_grid(ctx, F_DELTA, t.count)),   \
+                                       dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, ctx->xs[cur], ctx->xs[1 - cur],  \
+                                       ctx->x, ctx->v, ctx->rho, ctx->nb_cnt, ctx->nb_quads, fut, ctx->n,         \
+                                       t.count))
     if (last && fut) {
         PBF_LAUNCH_DELTA(true, true);
     } else if (last) {
@@ -877,9 +927,9 @@ hipError_t launch_viscosity(pbf_ctx* ctx) {
     track_begin(ctx, "applyViscosity");
     const BlockTable& t = table_for(ctx, F_VISC);
     PBF_DISPATCH_BS(ctx->bs_log2[F_VISC],
-                    hipLaunchKernelGGL(k_viscosity<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d,
-                                       ctx->xs[ctx->xs_cur], ctx->v, ctx->v_alt, ctx->omega, ctx->nb_cnt,
-                                       ctx->nb_quads, ctx->n));
+                    hipLaunchKernelGGL(k_viscosity<BS>, dim3(launch_grid(ctx, F_VISC, t.count)), dim3(BS), 0,
+                                       ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->v, ctx->v_alt,
+                                       ctx->omega, ctx->nb_cnt, ctx->nb_quads, ctx->n, t.count));
     track_end(ctx);
     float4* tmp = ctx->v;
     ctx->v = ctx->v_alt;
@@ -891,8 +941,9 @@ hipError_t launch_vorticity(pbf_ctx* ctx) {
     track_begin(ctx, "applyVorticity");
     const BlockTable& t = table_for(ctx, F_VORT);
     PBF_DISPATCH_BS(ctx->bs_log2[F_VORT],
-                    hipLaunchKernelGGL(k_vorticity<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d,
-                                       ctx->xs[ctx->xs_cur], ctx->v, ctx->omega, ctx->nb_cnt, ctx->nb_quads, ctx->n));
+                    hipLaunchKernelGGL(k_vorticity<BS>, dim3(launch_grid(ctx, F_VORT, t.count)), dim3(BS), 0,
+                                       ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->v, ctx->omega,
+                                       ctx->nb_cnt, ctx->nb_quads, ctx->n, t.count));
     track_end(ctx);
     return hipGetLastError();
 }

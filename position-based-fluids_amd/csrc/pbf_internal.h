@@ -95,6 +95,12 @@ struct pbf_ctx {
     pbf::BlockTable bt_predict;       // 256*kPredictItems particles per block
     int bs_log2[pbf::F_COUNT] = {8, 8, 8, 8, 8};  // chosen workgroup sizes
     bool friends_staged = false;      // friends build: LDS-staged candidates or global reads
+    // tiles per workgroup of each family: 1 = one workgroup per tile, k > 1 =
+    // workgroups striding over k tiles each, 0 = a persistent grid of
+    // resident[f][size] workgroups (the device's capacity for that kernel)
+    uint32_t tiles_per_wg[pbf::F_COUNT] = {1, 1, 1, 1, 1};
+    uint32_t resident[pbf::F_COUNT][4] = {};
+    int cu_count = 0;
 
     // particle arrays (current cell-sorted order)
     float4* x = nullptr;        // positions, w = |v|
@@ -157,6 +163,8 @@ hipError_t launch_sort(pbf_ctx* ctx);
 hipError_t launch_gather(pbf_ctx* ctx);
 hipError_t launch_cells(pbf_ctx* ctx);
 hipError_t launch_friends(pbf_ctx* ctx);
+// fills ctx->resident from the kernels' occupancy on ctx->cu_count CUs
+hipError_t query_residency(pbf_ctx* ctx);
 // aux / fut: per-particle records for the adjoint (nullptr: plain step)
 hipError_t launch_lambda(pbf_ctx* ctx, int iter, float4* aux = nullptr);
 hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last, float4* fut = nullptr);

@@ -137,7 +137,8 @@ class PbfStats(ctypes.Structure):
         ("unstaged_tiles", ctypes.c_uint32),
         ("workgroup_size", ctypes.c_uint32 * 5),
         ("friends_staged", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 7),
+        ("tiles_per_workgroup", ctypes.c_uint32 * 5),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -176,6 +177,7 @@ _SIGNATURES = {
     "pbf_reset_timings": (ctypes.c_int, [_VP]),
     "pbf_set_workgroup_sizes": (ctypes.c_int, [_VP, _UP]),
     "pbf_set_friends_mode": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "pbf_set_tiles_per_workgroup": (ctypes.c_int, [_VP, _UP]),
     "pbf_autotune": (ctypes.c_int, [_VP, ctypes.c_uint32, _UP]),
     "pbf_stats": (ctypes.c_int, [_VP, ctypes.POINTER(PbfStats)]),
     "pbf_read_log": (ctypes.c_int, [_VP, ctypes.POINTER(PbfLogMsg), ctypes.c_uint32, _UP,

@@ -176,8 +176,17 @@ class Simulation:
         self._ensure()
         check(lib().pbf_set_friends_mode(self._ctx, 1 if staged else 0), self._ctx)
 
+    def set_tiles_per_workgroup(self, tiles: dict):
+        """Tiles each workgroup of a family handles in turn: 1 (one per tile),
+        k > 1, or 0 for a persistent grid of resident workgroups."""
+        self._ensure()
+        arr = (ctypes.c_uint32 * len(FAMILIES))(*[int(tiles[f]) for f in FAMILIES])
+        check(lib().pbf_set_tiles_per_workgroup(self._ctx, arr), self._ctx)
+
     def autotune(self, steps: int = 3) -> dict:
-        """Times every workgroup size per kernel family; keeps the fastest."""
+        """Times every workgroup size per kernel family, then 4 tiles per
+        workgroup and a persistent grid; keeps the fastest.  Returns the
+        sizes (stats() reports the tiles per workgroup)."""
         self._ensure()
         arr = (ctypes.c_uint32 * len(FAMILIES))()
         check(lib().pbf_autotune(self._ctx, steps, arr), self._ctx)
@@ -187,8 +196,10 @@ class Simulation:
         self._ensure()
         st = PbfStats()
         check(lib().pbf_stats(self._ctx, ctypes.byref(st)), self._ctx)
-        out = {name: getattr(st, name) for name, _ in PbfStats._fields_ if name not in ("workgroup_size", "reserved")}
+        out = {name: getattr(st, name) for name, _ in PbfStats._fields_
+               if name not in ("workgroup_size", "tiles_per_workgroup", "reserved")}
         out["workgroup_size"] = dict(zip(FAMILIES, (int(a) for a in st.workgroup_size)))
+        out["tiles_per_workgroup"] = dict(zip(FAMILIES, (int(a) for a in st.tiles_per_workgroup)))
         return out
 
     def error_flags(self, reset: bool = False) -> tuple[int, int]:

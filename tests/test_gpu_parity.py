@@ -461,6 +461,52 @@ def test_workgroup_sizes_do_not_change_results():
             assert np.array_equal(outs[0][k].view(np.uint32), o[k].view(np.uint32)), k
 
 
+@pytest.mark.parametrize("tiles", [2, 4, 0])
+def test_tiles_per_workgroup_do_not_change_results(tiles):
+    """Strided (2, 4 tiles per workgroup) and persistent (0) launches of the
+    friends build and the sweeps give the one-tile-per-workgroup results bit
+    for bit, for one scene and for a batch of uneven scenes."""
+    fams = pbf_amd._lib.FAMILIES
+    scene = pbf_amd.config("dam16k", seed=9)
+    scenes = pbf_amd.ensemble(5, seed=4, n_min=700, n_max=20000)
+    for make, x0, v0 in (
+        (lambda: pbf_amd.Simulation(scene.params, scene.options), scene.x, scene.v),
+        (lambda: pbf_amd.BatchSimulation([s.params for s in scenes], [s.options for s in scenes]),
+         np.concatenate([s.x for s in scenes]), np.concatenate([s.v for s in scenes])),
+    ):
+        outs = []
+        for t in (1, tiles):
+            sim = make()
+            sim.set_state(x0, v0)
+            sim.set_workgroup_sizes({f: 64 for f in fams})   # many tiles per launch
+            sim.set_tiles_per_workgroup({f: t for f in fams})
+            assert sim.stats()["tiles_per_workgroup"] == {f: t for f in fams}
+            sim.run(4)
+            outs.append(sim.get_state())
+            sim.close()
+        for k in ("x", "v", "xs", "omega"):
+            assert np.array_equal(outs[0][k].view(np.uint32), outs[1][k].view(np.uint32)), (tiles, k)
+
+
+def test_persistent_grid_strides_at_1m():
+    """At 1M particles and 64-thread workgroups the persistent grid is
+    smaller than the 16,384 tiles, so every workgroup walks several tiles of
+    its XCD's run; results equal one workgroup per tile bit for bit."""
+    fams = pbf_amd._lib.FAMILIES
+    scene = pbf_amd.config("dam1m", seed=0)
+    outs = []
+    for t in (1, 0):
+        sim = pbf_amd.Simulation(scene.params, scene.options)
+        sim.set_state(scene.x, scene.v)
+        sim.set_workgroup_sizes({f: 64 for f in fams})
+        sim.set_tiles_per_workgroup({f: t for f in fams})
+        sim.run(2)
+        outs.append(sim.get_state())
+        sim.close()
+    for k in ("x", "v", "xs", "omega"):
+        assert np.array_equal(outs[0][k].view(np.uint32), outs[1][k].view(np.uint32)), k
+
+
 def test_autotune_restores_state_and_picks_sizes():
     scene = pbf_amd.config("dam16k", seed=8)
     ref_sim = pbf_amd.Simulation(scene.params, scene.options)
@@ -474,6 +520,7 @@ def test_autotune_restores_state_and_picks_sizes():
     assert set(chosen) == set(pbf_amd._lib.FAMILIES)
     assert all(v in (64, 128, 256, 512) for v in chosen.values())
     assert sim.stats()["workgroup_size"] == chosen
+    assert all(t in (0, 1, 4) for t in sim.stats()["tiles_per_workgroup"].values())
     sim.run(3)
     got = sim.get_state()
     for k in ("x", "v", "xs"):

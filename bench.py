@@ -39,9 +39,14 @@ STAGE_BYTES = {
 
 # HBM bytes per launch measured with rocprofv3 PMC counters on the
 # 1,048,576-particle dam break (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a
-# plain 256 MiB copy gives the two counters), averaged over the K launches of
-# a substep; profiles/pmc_traffic_dam1m.md holds the counts and commands.
-PMC_TRAFFIC_BYTES_PER_LAUNCH = {"computeDelta": 183.2e6, "computeScaling": 179.4e6}
+# plain 256 MiB copy gives the two counters), averaged over the workgroup
+# sizes and, for computeDelta, over the K launches of a substep (the last one
+# also writes v and x); profiles/pmc_traffic_dam1m.md holds the counts and
+# commands.
+PMC_TRAFFIC_BYTES_PER_LAUNCH = {
+    "predictPositions": 44.4e6, "onesweep": 20.4e6, "sortParticles": 90.8e6, "updateCells": 5.9e6,
+    "computeScaling": 185.3e6, "computeDelta": 189.7e6, "applyViscosity": 206.5e6, "applyVorticity": 226.7e6,
+}
 
 
 def kernel_bytes_per_particle(name: str, iters: int, passes: int) -> float | None:
@@ -363,7 +368,7 @@ def main() -> int:
     sim.set_state(scene.x, scene.v)
     sim.run(args.warmup)
     # tuned on the warmed-up state, which autotune restores afterwards
-    sizes = sim.autotune(2) if not args.no_autotune else sim.stats()["workgroup_size"]
+    sizes = sim.autotune(4) if not args.no_autotune else sim.stats()["workgroup_size"]
     # the timed region: substeps replayed as HIP graphs, no events between launches
     sim.set_graph(True)
     sim.run(2)
@@ -411,6 +416,25 @@ def main() -> int:
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     step_ms = {k: round(v / prof_steps, 4) for k, v in sorted(fam_ms.items(), key=lambda kv: -kv[1])}
 
+    # the same figures for every family with a byte model, with the HBM rate
+    # that the PMC traffic gives at the measured launch time
+    by_kernel = {}
+    for fam_name in measured:
+        ls = [(name, t) for name, t in timings.items() if family(name) == fam_name]
+        calls = sum(t["calls"] for _, t in ls)
+        ms = sum(t["sum_ms"] for _, t in ls) / max(calls, 1)
+        alg = sum(kernel_bytes_per_particle(name, iters, passes) * n * t["calls"] for name, t in ls) / max(calls, 1)
+        traffic = PMC_TRAFFIC_BYTES_PER_LAUNCH.get(fam_name) if n == 1048576 else None
+        by_kernel[fam_name] = {
+            "avg_launch_ms": round(ms, 5),
+            "algorithmic_bytes_per_launch": alg,
+            "achieved": alg / (ms * 1e-3) / 1e9,
+            "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "traffic_gbs": traffic / (ms * 1e-3) / 1e9 if traffic else None,
+            "traffic_frac": traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+        }
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -445,6 +469,7 @@ def main() -> int:
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
             "top_kernel_overall": top,
+            "by_kernel": by_kernel,
         },
         "stage_ms_per_step": step_ms,
         "error_flags": flags,

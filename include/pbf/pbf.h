@@ -196,8 +196,10 @@ int pbf_set_tiles_per_workgroup(pbf_ctx *ctx, const uint32_t *tiles /* PBF_FAMIL
 /* Autotuner: times every candidate size for each family (and both friends
  * build variants), then 4 tiles per workgroup and the persistent grid at the
  * chosen sizes, over `steps` substeps from the current state (which is
- * restored afterwards), keeps the fastest, and returns the chosen sizes in
- * `chosen` (may be NULL; pbf_stats reports the tiles per workgroup). */
+ * restored afterwards).  A choice other than the default (256, global-read
+ * friends build, one tile per workgroup) is kept only when it is more than
+ * 3% faster.  Returns the chosen sizes in `chosen` (may be NULL; pbf_stats
+ * reports the variant and the tiles per workgroup). */
 int pbf_autotune(pbf_ctx *ctx, uint32_t steps, uint32_t *chosen /* PBF_FAMILIES */);
 
 typedef struct pbf_stats_t {

@@ -908,35 +908,48 @@ int pbf_autotune(pbf_ctx* ctx, uint32_t steps, uint32_t* chosen) {
         return PBF_OK;
     };
     // workgroup sizes (and the friends build variant), one tile per workgroup
-    double best_mode_ms[2] = {1e300, 1e300};
-    int best_mode_l[2] = {8, 8};
+    // A candidate replaces the default (256, global-read friends build, one
+    // tile per workgroup) only when it is more than kMargin faster: the
+    // trials are a few substeps long, and choices within their noise would
+    // otherwise change from run to run.
+    constexpr double kMargin = 0.97;
+    double ms_at[2][4][pbf::F_COUNT];   // [staged trial][size][family]
     for (int f = 0; f < pbf::F_COUNT; ++f) ctx->tiles_per_wg[f] = 1;
     for (int trial = 0; trial < 8 && rc == PBF_OK; ++trial) {
         const int l = 6 + (trial & 3);
         const bool staged_trial = trial >= 4;
         ctx->friends_staged = staged_trial;
         for (int f = 0; f < pbf::F_COUNT; ++f) ctx->bs_log2[f] = l;
-        double ms[pbf::F_COUNT];
-        if ((rc = trial_ms(ms)) != PBF_OK) break;
-        for (int f = 0; f < pbf::F_COUNT; ++f) {
-            if (f == pbf::F_FRIENDS) {
-                if (ms[f] < best_mode_ms[staged_trial]) {
-                    best_mode_ms[staged_trial] = ms[f];
-                    best_mode_l[staged_trial] = l;
-                }
-            } else if (!staged_trial && ms[f] < best[f]) {   // the other families run in every trial
-                best[f] = ms[f];
-                best_l[f] = l;
-            }
-        }
+        rc = trial_ms(ms_at[staged_trial][l - 6]);
     }
-    ctx->friends_staged = best_mode_ms[1] < best_mode_ms[0];
-    best_l[pbf::F_FRIENDS] = best_mode_l[ctx->friends_staged ? 1 : 0];
-    best[pbf::F_FRIENDS] = best_mode_ms[ctx->friends_staged ? 1 : 0];
+    if (rc == PBF_OK) {
+        for (int f = 0; f < pbf::F_COUNT; ++f) {   // the fastest size that beats 256 by the margin
+            const double dflt = ms_at[0][2][f];
+            best_l[f] = 8;
+            best[f] = dflt;
+            for (int l = 6; l <= 9; ++l)
+                if (ms_at[0][l - 6][f] < kMargin * dflt && ms_at[0][l - 6][f] < best[f]) {
+                    best[f] = ms_at[0][l - 6][f];
+                    best_l[f] = l;
+                }
+        }
+        // the staged friends build against the chosen global-read one
+        ctx->friends_staged = false;
+        const double direct = best[pbf::F_FRIENDS];
+        for (int l = 6; l <= 9; ++l)
+            if (ms_at[1][l - 6][pbf::F_FRIENDS] < kMargin * direct &&
+                ms_at[1][l - 6][pbf::F_FRIENDS] < best[pbf::F_FRIENDS]) {
+                best[pbf::F_FRIENDS] = ms_at[1][l - 6][pbf::F_FRIENDS];
+                best_l[pbf::F_FRIENDS] = l;
+                ctx->friends_staged = true;
+            }
+    }
     for (int f = 0; f < pbf::F_COUNT; ++f) ctx->bs_log2[f] = best_l[f];
     // tiles per workgroup at those sizes: 4, then a persistent grid, against
     // one tile per workgroup (the staged friends build runs one tile each)
     uint32_t best_t[pbf::F_COUNT] = {1, 1, 1, 1, 1};
+    double one_tile[pbf::F_COUNT];
+    for (int f = 0; f < pbf::F_COUNT; ++f) one_tile[f] = best[f];
     static const uint32_t kTileTrials[2] = {4, 0};
     for (int k = 0; k < 2 && rc == PBF_OK; ++k) {
         for (int f = 0; f < pbf::F_COUNT; ++f) ctx->tiles_per_wg[f] = kTileTrials[k];
@@ -944,7 +957,7 @@ int pbf_autotune(pbf_ctx* ctx, uint32_t steps, uint32_t* chosen) {
         if ((rc = trial_ms(ms)) != PBF_OK) break;
         for (int f = 0; f < pbf::F_COUNT; ++f) {
             if (f == pbf::F_FRIENDS && ctx->friends_staged) continue;
-            if (ms[f] < best[f]) {
+            if (ms[f] < kMargin * one_tile[f] && ms[f] < best[f]) {
                 best[f] = ms[f];
                 best_t[f] = kTileTrials[k];
             }

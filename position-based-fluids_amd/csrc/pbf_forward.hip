@@ -366,7 +366,7 @@ __device__ __forceinline__ uint32_t cells_to_scan(const float4& p, int32_t cx, i
     return mask;
 }
 
-template <int BS>
+template <int BS, bool STRIDED>
 __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __restrict__ scenes,
                                                       const uint4* __restrict__ blocks, const float4* __restrict__ xs,
                                                       const uint2* __restrict__ cells, uint32_t* __restrict__ nb_cnt,
@@ -374,26 +374,28 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
                                                       uint32_t n, uint32_t fresh, uint32_t* err,
                                                       uint32_t ntiles) {
     __shared__ uint32_t s_stage[4 * kCircles * BS];
-    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
         const uint32_t i = be.y + threadIdx.x;
-        if (i >= be.z) continue;
-        uint4* list = reinterpret_cast<uint4*>(nb_list);
-        const float4 p = xs[i];
-        const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
-        FriendWriter<BS> w;
-        w.stage = s_stage + threadIdx.x;
-        w.bind(i, list, quads);
-        const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz)
-                    if (scan & (1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1))))
-                        scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list,
-                                           quads, w);
-        w.finish(i, nb_cnt, list, quads, err);
-        pack_friends(i, w.packed, list, quads);
+        if (i < be.z) {
+            uint4* list = reinterpret_cast<uint4*>(nb_list);
+            const float4 p = xs[i];
+            const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
+            FriendWriter<BS> w;
+            w.stage = s_stage + threadIdx.x;
+            w.bind(i, list, quads);
+            const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
+            for (int dx = -1; dx <= 1; ++dx)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dz = -1; dz <= 1; ++dz)
+                        if (scan & (1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1))))
+                            scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list,
+                                               quads, w);
+            w.finish(i, nb_cnt, list, quads, err);
+            pack_friends(i, w.packed, list, quads);
+        }
+        if (!STRIDED) break;   // one workgroup per tile: a single pass
     }
 }
 
@@ -574,31 +576,33 @@ struct LambdaAcc {
 
 // TAPE also stores (sum of gradients, denominator) of every particle for the
 // adjoint (aux).
-template <int BS, bool TAPE>
+template <int BS, bool TAPE, bool STRIDED>
 __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
                                               float4* __restrict__ xs, const uint32_t* __restrict__ nb_cnt,
                                               const uint4* __restrict__ nb_list, float* __restrict__ rho,
                                               float* __restrict__ lambda, float4* __restrict__ aux, uint32_t n,
                                               uint32_t ntiles) {
-    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
         const uint32_t i = be.y + threadIdx.x;
-        if (i >= be.z) continue;
-        const float4 p = xs[i];
-        LambdaAcc a;
-        walk_friends<3, 5, true>(
-            i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
-            [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
-        const float dens = a.dens * c.poly6;
-        rho[i] = dens;
-        const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
-        const float C = (dens / c.rest_density) - 1.0f;
-        const float den = gsum / (c.rest_density * c.rest_density) + c.epsilon;
-        const float lam = -1.0f * C / den;
-        lambda[i] = lam;
-        reinterpret_cast<float*>(xs + i)[3] = lam;
-        if (TAPE) aux[i] = make_float4(a.gx, a.gy, a.gz, den);
+        if (i < be.z) {
+            const float4 p = xs[i];
+            LambdaAcc a;
+            walk_friends<3, 5, true>(
+                i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
+                [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
+            const float dens = a.dens * c.poly6;
+            rho[i] = dens;
+            const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
+            const float C = (dens / c.rest_density) - 1.0f;
+            const float den = gsum / (c.rest_density * c.rest_density) + c.epsilon;
+            const float lam = -1.0f * C / den;
+            lambda[i] = lam;
+            reinterpret_cast<float*>(xs + i)[3] = lam;
+            if (TAPE) aux[i] = make_float4(a.gx, a.gy, a.gz, den);
+        }
+        if (!STRIDED) break;   // one workgroup per tile: a single pass
     }
 }
 
@@ -627,49 +631,51 @@ struct DeltaAcc {
 
 // TAPE also stores the position before the walls and which planes and quads
 // moved it (bits in w, as apply_walls returns them) for the adjoint (fut).
-template <int BS, bool LAST, bool TAPE>
+template <int BS, bool LAST, bool TAPE, bool STRIDED>
 __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
                                              const float4* __restrict__ xs, float4* __restrict__ xs_next,
                                              float4* __restrict__ x, float4* __restrict__ v,
                                              const float* __restrict__ rho, const uint32_t* __restrict__ nb_cnt,
                                              const uint4* __restrict__ nb_list, float4* __restrict__ fut, uint32_t n,
                                              uint32_t ntiles) {
-    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
         const uint32_t i = be.y + threadIdx.x;
-        if (i >= be.z) continue;
-        const float4 p = xs[i];
-        DeltaAcc a;
-        walk_friends<1, 2, true>(
-            i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
-            [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
-        float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
-        float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
-        float fz = p.z + (-c.spiky * a.sz) / c.rest_density;
-        const float4 prev = x[i];
-        const float ux = fx, uy = fy, uz = fz;
-        uint32_t hits = 0;
-        if (c.walls | c.quad_count) hits = apply_walls(fx, fy, fz, prev, c);
-        if (TAPE) fut[i] = make_float4(ux, uy, uz, __uint_as_float(hits));
-        const float dx = fx - p.x, dy = fy - p.y, dz = fz - p.z;
-        float4 o;
-        o.x = p.x + dx;
-        o.y = p.y + dy;
-        o.z = p.z + dz;
-        if (!LAST) {
-            o.w = p.w;
-            xs_next[i] = o;
-        } else {
-            o.w = rho[i];
-            xs_next[i] = o;
-            const float vx = (o.x - prev.x) / c.dt, vy = (o.y - prev.y) / c.dt, vz = (o.z - prev.z) / c.dt;
-            float* vp = reinterpret_cast<float*>(v + i);
-            vp[0] = vx;
-            vp[1] = vy;
-            vp[2] = vz;
-            x[i] = make_float4(o.x, o.y, o.z, sqrtf(vx * vx + vy * vy + vz * vz));
+        if (i < be.z) {
+            const float4 p = xs[i];
+            DeltaAcc a;
+            walk_friends<1, 2, true>(
+                i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
+                [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
+            float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
+            float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
+            float fz = p.z + (-c.spiky * a.sz) / c.rest_density;
+            const float4 prev = x[i];
+            const float ux = fx, uy = fy, uz = fz;
+            uint32_t hits = 0;
+            if (c.walls | c.quad_count) hits = apply_walls(fx, fy, fz, prev, c);
+            if (TAPE) fut[i] = make_float4(ux, uy, uz, __uint_as_float(hits));
+            const float dx = fx - p.x, dy = fy - p.y, dz = fz - p.z;
+            float4 o;
+            o.x = p.x + dx;
+            o.y = p.y + dy;
+            o.z = p.z + dz;
+            if (!LAST) {
+                o.w = p.w;
+                xs_next[i] = o;
+            } else {
+                o.w = rho[i];
+                xs_next[i] = o;
+                const float vx = (o.x - prev.x) / c.dt, vy = (o.y - prev.y) / c.dt, vz = (o.z - prev.z) / c.dt;
+                float* vp = reinterpret_cast<float*>(v + i);
+                vp[0] = vx;
+                vp[1] = vy;
+                vp[2] = vz;
+                x[i] = make_float4(o.x, o.y, o.z, sqrtf(vx * vx + vy * vy + vz * vz));
+            }
         }
+        if (!STRIDED) break;   // one workgroup per tile: a single pass
     }
 }
 
@@ -698,30 +704,32 @@ struct ViscAcc {
     }
 };
 
-template <int BS>
+template <int BS, bool STRIDED>
 __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict__ scenes,
                                                  const uint4* __restrict__ blocks, const float4* __restrict__ xs,
                                                  const float4* __restrict__ v_in, float4* __restrict__ v_out,
                                                  float4* __restrict__ omega, const uint32_t* __restrict__ nb_cnt,
                                                  const uint4* __restrict__ nb_list, uint32_t n, uint32_t ntiles) {
-    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
         const uint32_t i = be.y + threadIdx.x;
-        if (i >= be.z) continue;
-        const float4 p = xs[i];
-        const float4 vi = v_in[i];
-        ViscAcc a;
-        walk_friends<1, 2, true>(
-            i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], v_in[j]}; },
-            [&](const Pair4& t, bool valid) { a.add(p, vi, t.a, t.b, c, valid); });
-        float4 vo;
-        vo.x = vi.x + c.viscosity * (a.vsx * c.poly6);
-        vo.y = vi.y + c.viscosity * (a.vsy * c.poly6);
-        vo.z = vi.z + c.viscosity * (a.vsz * c.poly6);
-        vo.w = vi.w + c.viscosity * 0.0f;
-        v_out[i] = vo;
-        omega[i] = make_float4(a.ox * -c.spiky, a.oy * -c.spiky, a.oz * -c.spiky, 0.0f);
+        if (i < be.z) {
+            const float4 p = xs[i];
+            const float4 vi = v_in[i];
+            ViscAcc a;
+            walk_friends<1, 2, true>(
+                i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], v_in[j]}; },
+                [&](const Pair4& t, bool valid) { a.add(p, vi, t.a, t.b, c, valid); });
+            float4 vo;
+            vo.x = vi.x + c.viscosity * (a.vsx * c.poly6);
+            vo.y = vi.y + c.viscosity * (a.vsy * c.poly6);
+            vo.z = vi.z + c.viscosity * (a.vsz * c.poly6);
+            vo.w = vi.w + c.viscosity * 0.0f;
+            v_out[i] = vo;
+            omega[i] = make_float4(a.ox * -c.spiky, a.oy * -c.spiky, a.oz * -c.spiky, 0.0f);
+        }
+        if (!STRIDED) break;   // one workgroup per tile: a single pass
     }
 }
 
@@ -744,34 +752,36 @@ struct VortAcc {
     }
 };
 
-template <int BS>
+template <int BS, bool STRIDED>
 __global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict__ scenes,
                                                  const uint4* __restrict__ blocks, const float4* __restrict__ xs,
                                                  float4* __restrict__ v, const float4* __restrict__ omega,
                                                  const uint32_t* __restrict__ nb_cnt,
                                                  const uint4* __restrict__ nb_list, uint32_t n, uint32_t ntiles) {
-    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // one tile per pass
+    for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
         const uint32_t i = be.y + threadIdx.x;
-        if (i >= be.z) continue;
-        const float4 p = xs[i];
-        VortAcc a;
-        walk_friends<1, 2, true>(
-            i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], omega[j]}; },
-            [&](const Pair4& t, bool valid) { a.add(p, t.a, t.b, c, valid); });
-        float ex = a.ex * -c.spiky, ey = a.ey * -c.spiky, ez = a.ez * -c.spiky;
-        const float l = sqrtf(ex * ex + ey * ey + ez * ez);
-        if (l > 0.0f) { ex /= l; ey /= l; ez /= l; }
-        const float4 oi = omega[i];
-        const float cx = ey * oi.z - ez * oi.y;
-        const float cy = ez * oi.x - ex * oi.z;
-        const float cz = ex * oi.y - ey * oi.x;
-        float4 vi = v[i];
-        vi.x += (c.vorticity * cx) * c.dt;
-        vi.y += (c.vorticity * cy) * c.dt;
-        vi.z += (c.vorticity * cz) * c.dt;
-        v[i] = vi;
+        if (i < be.z) {
+            const float4 p = xs[i];
+            VortAcc a;
+            walk_friends<1, 2, true>(
+                i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], omega[j]}; },
+                [&](const Pair4& t, bool valid) { a.add(p, t.a, t.b, c, valid); });
+            float ex = a.ex * -c.spiky, ey = a.ey * -c.spiky, ez = a.ez * -c.spiky;
+            const float l = sqrtf(ex * ex + ey * ey + ez * ez);
+            if (l > 0.0f) { ex /= l; ey /= l; ez /= l; }
+            const float4 oi = omega[i];
+            const float cx = ey * oi.z - ez * oi.y;
+            const float cy = ez * oi.x - ex * oi.z;
+            const float cz = ex * oi.y - ey * oi.x;
+            float4 vi = v[i];
+            vi.x += (c.vorticity * cx) * c.dt;
+            vi.y += (c.vorticity * cy) * c.dt;
+            vi.z += (c.vorticity * cz) * c.dt;
+            v[i] = vi;
+        }
+        if (!STRIDED) break;   // one workgroup per tile: a single pass
     }
 }
 
@@ -809,11 +819,11 @@ uint32_t resident_blocks(const pbf_ctx* ctx, K kernel, int bs) {
 hipError_t query_residency(pbf_ctx* ctx) {
     for (int l = 6; l <= 9; ++l) {
         PBF_DISPATCH_BS(l, {
-            ctx->resident[F_FRIENDS][l - 6] = resident_blocks(ctx, k_friends_direct<BS>, BS);
-            ctx->resident[F_LAMBDA][l - 6] = resident_blocks(ctx, k_lambda<BS, false>, BS);
-            ctx->resident[F_DELTA][l - 6] = resident_blocks(ctx, k_delta<BS, false, false>, BS);
-            ctx->resident[F_VISC][l - 6] = resident_blocks(ctx, k_viscosity<BS>, BS);
-            ctx->resident[F_VORT][l - 6] = resident_blocks(ctx, k_vorticity<BS>, BS);
+            ctx->resident[F_FRIENDS][l - 6] = resident_blocks(ctx, k_friends_direct<BS, true>, BS);
+            ctx->resident[F_LAMBDA][l - 6] = resident_blocks(ctx, k_lambda<BS, false, true>, BS);
+            ctx->resident[F_DELTA][l - 6] = resident_blocks(ctx, k_delta<BS, false, false, true>, BS);
+            ctx->resident[F_VISC][l - 6] = resident_blocks(ctx, k_viscosity<BS, true>, BS);
+            ctx->resident[F_VORT][l - 6] = resident_blocks(ctx, k_vorticity<BS, true>, BS);
         });
     }
     return hipGetLastError();
@@ -860,6 +870,22 @@ hipError_t launch_cells(pbf_ctx* ctx) {
     return hipGetLastError();
 }
 
+// One launch of a tiled kernel: KERNEL<..., STRIDED> over the family's
+// block table, single-pass when the grid has one workgroup per tile.  KERNEL
+// is the template name and its leading arguments without the closing '>'
+// (COMMA separates them).
+#define COMMA ,
+#define PBF_LAUNCH_TILED(FAM, KERNEL, ...)                                                                     \
+    do {                                                                                                       \
+        const uint32_t grid_ = launch_grid(ctx, FAM, t.count);                                                 \
+        PBF_DISPATCH_BS(ctx->bs_log2[FAM], {                                                                   \
+            if (grid_ < t.count)                                                                               \
+                hipLaunchKernelGGL((KERNEL, true>), dim3(grid_), dim3(BS), 0, ctx->stream, __VA_ARGS__, t.count);  \
+            else                                                                                               \
+                hipLaunchKernelGGL((KERNEL, false>), dim3(grid_), dim3(BS), 0, ctx->stream, __VA_ARGS__, t.count); \
+        });                                                                                                    \
+    } while (0)
+
 hipError_t launch_friends(pbf_ctx* ctx) {
     track_begin(ctx, "buildFriendsList");
     const uint32_t fresh = ctx->cells_fresh ? 1u : 0u;
@@ -870,11 +896,9 @@ hipError_t launch_friends(pbf_ctx* ctx) {
                                            ctx->xs[ctx->xs_cur], reinterpret_cast<const uint2*>(ctx->cells),
                                            ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, fresh, ctx->err));
     } else {
-        PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS],
-                        hipLaunchKernelGGL(k_friends_direct<BS>, dim3(launch_grid(ctx, F_FRIENDS, t.count)), dim3(BS),
-                                           0, ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur],
-                                           reinterpret_cast<const uint2*>(ctx->cells), ctx->nb_cnt, ctx->nb_list,
-                                           ctx->nb_quads, ctx->n, fresh, ctx->err, t.count));
+        PBF_LAUNCH_TILED(F_FRIENDS, k_friends_direct<BS, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur],
+                         reinterpret_cast<const uint2*>(ctx->cells), ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n,
+                         fresh, ctx->err);
     }
     track_end(ctx);
     return hipGetLastError();
@@ -883,17 +907,12 @@ hipError_t launch_friends(pbf_ctx* ctx) {
 hipError_t launch_lambda(pbf_ctx* ctx, int iter, float4* aux) {
     track_begin(ctx, "computeScaling", iter);
     const BlockTable& t = table_for(ctx, F_LAMBDA);
-    if (aux) {
-        PBF_DISPATCH_BS(ctx->bs_log2[F_LAMBDA],
-                        hipLaunchKernelGGL((k_lambda<BS, true>), dim3(launch_grid(ctx, F_LAMBDA, t.count)), dim3(BS),
-                                           0, ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt,
-                                           ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n, t.count));
-    } else {
-        PBF_DISPATCH_BS(ctx->bs_log2[F_LAMBDA],
-                        hipLaunchKernelGGL((k_lambda<BS, false>), dim3(launch_grid(ctx, F_LAMBDA, t.count)),
-                                           dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur],
-                                           ctx->nb_cnt, ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n, t.count));
-    }
+    if (aux)
+        PBF_LAUNCH_TILED(F_LAMBDA, k_lambda<BS COMMA true, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt,
+                         ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n);
+    else
+        PBF_LAUNCH_TILED(F_LAMBDA, k_lambda<BS COMMA false, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt,
+                         ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n);
     track_end(ctx);
     return hipGetLastError();
 }
@@ -902,12 +921,9 @@ hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last, float4* fut) {
     track_begin(ctx, "computeDelta", iter);
     const int cur = ctx->xs_cur;
     const BlockTable& t = table_for(ctx, F_DELTA);
-#define PBF_LAUNCH_DELTA(LASTV, TAPEV)                                                                            \
-    PBF_DISPATCH_BS(ctx->bs_log2[F_DELTA],                                                                       \
-                    hipLaunchKernelGGL((k_delta<BS, LASTV, TAPEV>), dim3(launch_grid(ctx, F_DELTA, t.count)),   \
-                                       dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, ctx->xs[cur], ctx->xs[1 - cur],  \
-                                       ctx->x, ctx->v, ctx->rho, ctx->nb_cnt, ctx->nb_quads, fut, ctx->n,         \
-                                       t.count))
+#define PBF_LAUNCH_DELTA(LASTV, TAPEV)                                                                           \
+    PBF_LAUNCH_TILED(F_DELTA, k_delta<BS COMMA LASTV COMMA TAPEV, ctx->sc_d, t.d, ctx->xs[cur], ctx->xs[1 - cur], \
+                     ctx->x, ctx->v, ctx->rho, ctx->nb_cnt, ctx->nb_quads, fut, ctx->n)
     if (last && fut) {
         PBF_LAUNCH_DELTA(true, true);
     } else if (last) {
@@ -926,10 +942,8 @@ hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last, float4* fut) {
 hipError_t launch_viscosity(pbf_ctx* ctx) {
     track_begin(ctx, "applyViscosity");
     const BlockTable& t = table_for(ctx, F_VISC);
-    PBF_DISPATCH_BS(ctx->bs_log2[F_VISC],
-                    hipLaunchKernelGGL(k_viscosity<BS>, dim3(launch_grid(ctx, F_VISC, t.count)), dim3(BS), 0,
-                                       ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->v, ctx->v_alt,
-                                       ctx->omega, ctx->nb_cnt, ctx->nb_quads, ctx->n, t.count));
+    PBF_LAUNCH_TILED(F_VISC, k_viscosity<BS, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->v, ctx->v_alt, ctx->omega,
+                     ctx->nb_cnt, ctx->nb_quads, ctx->n);
     track_end(ctx);
     float4* tmp = ctx->v;
     ctx->v = ctx->v_alt;
@@ -940,12 +954,13 @@ hipError_t launch_viscosity(pbf_ctx* ctx) {
 hipError_t launch_vorticity(pbf_ctx* ctx) {
     track_begin(ctx, "applyVorticity");
     const BlockTable& t = table_for(ctx, F_VORT);
-    PBF_DISPATCH_BS(ctx->bs_log2[F_VORT],
-                    hipLaunchKernelGGL(k_vorticity<BS>, dim3(launch_grid(ctx, F_VORT, t.count)), dim3(BS), 0,
-                                       ctx->stream, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->v, ctx->omega,
-                                       ctx->nb_cnt, ctx->nb_quads, ctx->n, t.count));
+    PBF_LAUNCH_TILED(F_VORT, k_vorticity<BS, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->v, ctx->omega, ctx->nb_cnt,
+                     ctx->nb_quads, ctx->n);
     track_end(ctx);
     return hipGetLastError();
 }
+
+#undef PBF_LAUNCH_TILED
+#undef COMMA
 
 }  // namespace pbf

@@ -14,7 +14,7 @@ namespace pbf {
 
 // Onesweep radix sort geometry: 256 threads, 8 keys per thread, 8-bit digits.
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 8;
+constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;

@@ -8,7 +8,7 @@
 // Design (one kernel launch per 8-bit digit pass, no host syncs):
 //  * the digit histograms of all passes are accumulated by the predict/hash
 //    kernel (pbf_forward.hip), so each pass knows every digit's global base;
-//  * each workgroup claims a tile of 2048 keys in launch order through an
+//  * each workgroup claims a tile of 4096 keys in launch order through an
 //    atomic counter, ranks its keys stably with wave64 ballot matching
 //    (8 ballots per key give the lanes holding the same digit), and learns the
 //    keys of the same digit in earlier tiles by decoupled look-back over a

@@ -30,7 +30,7 @@ constexpr int kErrWords = 8;
 // end, 0} per workgroup, so a workgroup never straddles two scenes.  Tables
 // exist for these workgroup sizes (index = log2(size) - 6).
 constexpr int kNumBlockSizes = 5;  // 64, 128, 256, 512, 1024
-constexpr int kPredictItems = 4;   // particles per thread in k_predict_hash
+constexpr int kPredictItems = 8;   // particles per thread in k_predict_hash
 
 struct BlockTable {
     uint4* d = nullptr;

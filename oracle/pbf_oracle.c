@@ -539,7 +539,10 @@ void pbfo_viscosity(const pbfo_cfg *c, const float *xs, const float *v_in, const
         for (int k = 0; k < 3; ++k) vo[k] = vi[k] + c->viscosity * (vs[k] * poly6);
         vo[3] = vi[3] + c->viscosity * 0.0f;
         for (int k = 0; k < 3; ++k) oo[k] = om[k] * -spiky;
-        oo[3] = 0.0f;
+        /* w = |omega| where the reference stores 0 (apply_viscosity.cl:86):
+         * the GPU's vorticity sweep reads it instead of a length per pair;
+         * the vorticity step below still takes |omega_j| from x, y, z */
+        oo[3] = pair_rl(dot3(oo, oo));
     }
 }
 

@@ -727,7 +727,11 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
             vo.z = vi.z + c.viscosity * (a.vsz * c.poly6);
             vo.w = vi.w + c.viscosity * 0.0f;
             v_out[i] = vo;
-            omega[i] = make_float4(a.ox * -c.spiky, a.oy * -c.spiky, a.oz * -c.spiky, 0.0f);
+            // w carries |omega| (the reference stores 0, apply_viscosity.cl:86),
+            // so the vorticity sweep reads each neighbour's length instead of
+            // taking a square root per pair
+            const float ox = a.ox * -c.spiky, oy = a.oy * -c.spiky, oz = a.oz * -c.spiky;
+            omega[i] = make_float4(ox, oy, oz, norm_of_sq(ox * ox + oy * oy + oz * oz));
         }
         if (!STRIDED) break;   // one workgroup per tile: a single pass
     }
@@ -745,7 +749,7 @@ struct VortAcc {
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
-        const float wl = norm_of_sq(oj.x * oj.x + oj.y * oj.y + oj.z * oj.z);
+        const float wl = oj.w;   // |omega_j| (fast_length, apply_vorticity.cl:63), stored by k_viscosity
         ex += masked(wl * ax, in);
         ey += masked(wl * ay, in);
         ez += masked(wl * az, in);

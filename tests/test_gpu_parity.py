@@ -479,6 +479,8 @@ def test_tiles_per_workgroup_do_not_change_results(tiles):
             sim = make()
             sim.set_state(x0, v0)
             sim.set_workgroup_sizes({f: 64 for f in fams})   # many tiles per launch
+            with pytest.raises(pbf_amd.PbfError):
+                sim.set_tiles_per_workgroup({f: 65 for f in fams})   # at most 64
             sim.set_tiles_per_workgroup({f: t for f in fams})
             assert sim.stats()["tiles_per_workgroup"] == {f: t for f in fams}
             sim.run(4)

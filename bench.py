@@ -55,10 +55,8 @@ def kernel_bytes_per_particle(name: str, iters: int, passes: int) -> float | Non
         return STAGE_BYTES["predict_hash"]
     if base == "onesweep":
         return (4 + 16 * passes) / passes
-    if base == "sortParticles":
-        return STAGE_BYTES["gather"]
-    if base == "updateCells":
-        return STAGE_BYTES["cells"]
+    if base == "sortParticles":   # k_gather_cells: sortParticles + updateCells
+        return STAGE_BYTES["gather"] + STAGE_BYTES["cells"]
     if base == "computeScaling":
         return STAGE_BYTES["lambda"]
     if base == "computeDelta":

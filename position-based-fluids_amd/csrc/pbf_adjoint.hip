@@ -821,13 +821,11 @@ int tape_step(pbf_adjoint* a, const TapeSlot& t, unsigned long long* note) {
     };
     ctx->pause_now = 0;
     ctx->cells_fresh = true;
-    hipError_t e = launch_clear(ctx);
-    if (e == hipSuccess) e = launch_predict_hash(ctx, ctx->cells_hold_keys);
+    hipError_t e = launch_predict_hash(ctx, ctx->cells_hold_keys);   // the sort workspace is zero between steps
     ctx->perm = t.perm;
     if (e == hipSuccess) e = launch_sort(ctx);
     ctx->xs[0] = t.P;
-    if (e == hipSuccess) e = launch_gather(ctx);
-    if (e == hipSuccess) e = launch_cells(ctx);
+    if (e == hipSuccess) e = launch_gather(ctx);   // + cells, and zeroes the sort workspace again
     ctx->cells_hold_keys = true;
     ctx->order_sorted = true;
     ctx->nb_cnt = t.cnt;

@@ -292,6 +292,7 @@ int create_impl(const pbf_params* params, const pbf_options* opts, uint32_t S, p
     CREATE_TRY(hipMemsetAsync(ctx->nb_cnt, 0, 4 * std::max<size_t>(n, 1), ctx->stream));
     CREATE_TRY(hipMemsetAsync(ctx->v, 0, 16 * std::max<size_t>(n, 1), ctx->stream));
     CREATE_TRY(hipMemsetAsync(ctx->err, 0, 4 * pbf::kErrAllocWords, ctx->stream));
+    CREATE_TRY(pbf::launch_clear(ctx));   // the sort workspace starts zero
     CREATE_TRY(hipStreamSynchronize(ctx->stream));
 #undef CREATE_TRY
     *out = ctx;
@@ -316,11 +317,11 @@ int enqueue_step(pbf_ctx* ctx, uint32_t pause) {
     if (pause && ctx->order_sorted) {
         HIP_TRY(ctx, pbf::launch_predict_only(ctx, 1u));
     } else {
-        HIP_TRY(ctx, pbf::launch_clear(ctx));
+        // the sort workspace is zero between steps: cleared at create, then
+        // by k_gather_cells after every sort
         HIP_TRY(ctx, pbf::launch_predict_hash(ctx, ctx->cells_hold_keys));
         HIP_TRY(ctx, pbf::launch_sort(ctx));
-        HIP_TRY(ctx, pbf::launch_gather(ctx));
-        HIP_TRY(ctx, pbf::launch_cells(ctx));
+        HIP_TRY(ctx, pbf::launch_gather(ctx));   // + updateCells
         ctx->cells_hold_keys = true;
         ctx->order_sorted = true;
     }

@@ -4,8 +4,8 @@
 //   k_predict_hash   predictPositions + computeKeys + digit histograms for the
 //                    sort + reset of the previous step's cells
 //   (onesweep passes, pbf_sort.hip)
-//   k_gather         sortParticles; recomputes x* from the gathered x, v
-//   k_cells          updateCells
+//   k_gather_cells   sortParticles (recomputes x* from the gathered x, v) +
+//                    updateCells; zeroes the sort workspace for the next step
 //   k_friends_direct buildFriendsList, candidates read from global memory
 //   / k_friends      (or from the workgroup's neighbourhood staged in LDS)
 //   K x { k_lambda   computeScaling + packData(lambda)
@@ -113,15 +113,24 @@ __global__ __launch_bounds__(256) void k_predict_only(const SceneConsts* __restr
     xs[i] = predicted_from(x[i], v[i], c, pause);
 }
 
-// radixsort.cl:22-37.  x* is recomputed from the gathered x and v with the
-// same operations as k_predict_hash, so it is bitwise the value that was
-// hashed; v itself is not permuted (it is rewritten before it is read).
-__global__ __launch_bounds__(256) void k_gather(const SceneConsts* __restrict__ scenes,
-                                               const uint4* __restrict__ blocks,
-                                               const uint32_t* __restrict__ perm, const float4* __restrict__ x,
-                                               const float4* __restrict__ v, float4* __restrict__ x_out,
-                                               float4* __restrict__ xs_out, uint32_t pause, uint32_t n,
-                                               uint32_t* err) {
+// radixsort.cl:22-37 (sortParticles) and update_cells.cl:1-26 (updateCells)
+// in one pass over the sorted order.  x* is recomputed from the gathered x
+// and v with the same operations as k_predict_hash, so it is bitwise the
+// value that was hashed; v itself is not permuted (it is rewritten before it
+// is read).  The cells come from the sorted global keys (the last particle
+// always closes its cell).  The launch also zeroes the sort workspace
+// (histograms, tile counters, look-back status), whose last reader was this
+// step's final sort pass, for the next step's histograms.
+__global__ __launch_bounds__(256) void k_gather_cells(const SceneConsts* __restrict__ scenes,
+                                                     const uint4* __restrict__ blocks,
+                                                     const uint32_t* __restrict__ perm, const float4* __restrict__ x,
+                                                     const float4* __restrict__ v, float4* __restrict__ x_out,
+                                                     float4* __restrict__ xs_out, const uint32_t* __restrict__ ks,
+                                                     uint32_t* __restrict__ cells, uint32_t total_cells,
+                                                     uint32_t* __restrict__ ws, size_t ws_words, uint32_t pause,
+                                                     uint32_t n, uint32_t* err) {
+    const size_t stride = (size_t)gridDim.x * 256u;
+    for (size_t w = (size_t)blockIdx.x * 256u + threadIdx.x; w < ws_words; w += stride) ws[w] = 0u;
     const uint4 be = blocks[blockIdx.x];
     const SceneConsts& c = scenes[be.x];
     const uint32_t i = be.y + threadIdx.x;
@@ -135,14 +144,7 @@ __global__ __launch_bounds__(256) void k_gather(const SceneConsts* __restrict__ 
     const float4 xp = x[p];
     x_out[i] = xp;
     xs_out[i] = predicted_from(xp, v[p], c, pause);
-}
 
-// update_cells.cl:1-26 over the sorted global keys (the last particle always
-// closes its cell).
-__global__ __launch_bounds__(256) void k_cells(const uint32_t* __restrict__ ks, uint32_t* __restrict__ cells,
-                                              uint32_t n, uint32_t total_cells, uint32_t* err) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
     const uint32_t k = ks[i];
     if (k >= total_cells || (i > 0 && ks[i - 1] >= total_cells)) {  // only after a failed sort
         atomicOr(err, (uint32_t)PBF_FLAG_INTERNAL);
@@ -859,21 +861,14 @@ hipError_t launch_predict_only(pbf_ctx* ctx, uint32_t pause) {
 hipError_t launch_gather(pbf_ctx* ctx) {
     track_begin(ctx, "sortParticles");
     const BlockTable& t = ctx->bt[2];
-    hipLaunchKernelGGL(k_gather, dim3(t.count), dim3(256), 0, ctx->stream, ctx->sc_d, t.d, ctx->perm, ctx->x, ctx->v,
-                       ctx->x_alt, ctx->xs[0], ctx->pause_now, ctx->n, ctx->err);
+    hipLaunchKernelGGL(k_gather_cells, dim3(t.count), dim3(256), 0, ctx->stream, ctx->sc_d, t.d, ctx->perm, ctx->x,
+                       ctx->v, ctx->x_alt, ctx->xs[0], ctx->keys_sorted, ctx->cells, ctx->total_cells, ctx->sort.ws,
+                       ctx->sort.ws_words, ctx->pause_now, ctx->n, ctx->err);
     track_end(ctx);
     float4* tmp = ctx->x;
     ctx->x = ctx->x_alt;
     ctx->x_alt = tmp;
     ctx->xs_cur = 0;
-    return hipGetLastError();
-}
-
-hipError_t launch_cells(pbf_ctx* ctx) {
-    track_begin(ctx, "updateCells");
-    hipLaunchKernelGGL(k_cells, dim3((ctx->n + 255) / 256), dim3(256), 0, ctx->stream, ctx->keys_sorted, ctx->cells,
-                       ctx->n, ctx->total_cells, ctx->err);
-    track_end(ctx);
     return hipGetLastError();
 }
 

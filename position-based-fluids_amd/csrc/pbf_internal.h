@@ -161,7 +161,6 @@ hipError_t launch_predict_hash(pbf_ctx* ctx, bool reset_old);
 hipError_t launch_predict_only(pbf_ctx* ctx, uint32_t pause);
 hipError_t launch_sort(pbf_ctx* ctx);
 hipError_t launch_gather(pbf_ctx* ctx);
-hipError_t launch_cells(pbf_ctx* ctx);
 hipError_t launch_friends(pbf_ctx* ctx);
 // fills ctx->resident from the kernels' occupancy on ctx->cu_count CUs
 hipError_t query_residency(pbf_ctx* ctx);

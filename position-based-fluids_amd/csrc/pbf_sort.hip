@@ -160,12 +160,12 @@ __global__ void k_clear_words(uint32_t* __restrict__ p, size_t words) {
 
 }  // namespace
 
+// Zeroes the sort workspace; run once at create (k_gather_cells zeroes it
+// after every sort).
 hipError_t launch_clear(pbf_ctx* ctx) {
     const size_t words = ctx->sort.ws_words;
     const unsigned blocks = (unsigned)((words + 255) / 256);
-    track_begin(ctx, "clearSort");
     hipLaunchKernelGGL(k_clear_words, dim3(blocks), dim3(256), 0, ctx->stream, ctx->sort.ws, words);
-    track_end(ctx);
     return hipGetLastError();
 }
 

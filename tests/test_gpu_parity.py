@@ -401,7 +401,7 @@ def test_profiling_trackers():
     sim.set_profiling(True)
     sim.run(3)
     t = sim.timings()
-    for name in ("predictPositions", "onesweep_0", "sortParticles", "updateCells", "buildFriendsList",
+    for name in ("predictPositions", "onesweep_0", "sortParticles", "buildFriendsList",
                  "computeScaling_0", "computeDelta_3", "applyViscosity", "applyVorticity"):
         assert name in t and t[name]["calls"] == 3 and t[name]["last_ms"] > 0.0, name
     sim.close()

@@ -44,8 +44,8 @@ STAGE_BYTES = {
 # also writes v and x); profiles/pmc_traffic_dam1m.md holds the counts and
 # commands.
 PMC_TRAFFIC_BYTES_PER_LAUNCH = {
-    "predictPositions": 44.4e6, "onesweep": 20.4e6, "sortParticles": 90.8e6, "updateCells": 5.9e6,
-    "computeScaling": 185.3e6, "computeDelta": 189.7e6, "applyViscosity": 206.5e6, "applyVorticity": 226.7e6,
+    "predictPositions": 43.9e6, "onesweep": 17.8e6, "sortParticles": 97.5e6,
+    "computeScaling": 185.3e6, "computeDelta": 190.0e6, "applyViscosity": 204.1e6, "applyVorticity": 202.5e6,
 }
 
 

@@ -22,8 +22,9 @@
 // particle's circle lists (counts packed 5 x 6 bits) and the list a sweep
 // processes, grouped per 64 consecutive particles in 16-byte quads, so a
 // wave's load of one quad is a contiguous 1 KB (pbf_device.h).  The sweeps
-// walk a particle's friends a quad at a time: four gathers in flight, then
-// the four updates in the reference's order.
+// walk a particle's friends a quad at a time, one quad ahead: the next
+// quad's four gathers are in flight while the current quad's four updates
+// run in the reference's order.
 #include <algorithm>
 
 #include "pbf_internal.h"
@@ -540,8 +541,8 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
 // ---------------------------------------------------------------------------
 // Friend sweeps.  Each walks the friends of the circles it processes in the
 // reference's order (circle-major, then scan order), four per 16-byte load
-// (for_friends, pbf_device.h): the four gathers are issued together, then the
-// updates run in order.
+// (walk_friends, pbf_device.h): the gathers of the next quad are issued
+// before the updates of the current one run in order.
 // ---------------------------------------------------------------------------
 
 // Two float4 of one friend (position and velocity or vorticity).

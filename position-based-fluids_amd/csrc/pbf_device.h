@@ -225,33 +225,10 @@ __device__ __host__ __forceinline__ uint32_t friend_at(const uint32_t* list, con
     return c == 0 ? sweep[sweep_quad(i, f >> 2) * 4 + (size_t)(f & 3)] : list[friend_slot(i, c, f)];
 }
 
-// Walks the first m entries of particle i's list in quad layout, four per
-// 16-byte load (fn4 on a full quad, fn1 on each entry of the partial last one).
-template <class Fn4, class Fn1>
-__device__ __forceinline__ void for_quads(uint32_t i, int m, const uint4* __restrict__ quads, Fn4&& fn4, Fn1&& fn1) {
-    int q = 0;
-    for (; 4 * q + 4 <= m; ++q) fn4(quads[sweep_quad(i, q)]);
-    const int rest = m - 4 * q;
-    if (rest > 0) {
-        const uint4 jj = quads[sweep_quad(i, q)];
-        fn1(jj.x);
-        if (rest > 1) fn1(jj.y);
-        if (rest > 2) fn1(jj.z);
-    }
-}
-
-// The friends a sweep with threshold L_NUM/L_DEN processes.
-template <int L_NUM, int L_DEN, class Fn4, class Fn1>
-__device__ __forceinline__ void for_friends(uint32_t i, uint32_t packed, const uint4* __restrict__ quads, Fn4&& fn4,
-                                            Fn1&& fn1) {
-    int m;
-    (void)circles_processed<L_NUM, L_DEN>(packed, &m);
-    for_quads(i, m, quads, fn4, fn1);
-}
-
-// The same walk in two phases per quad, for the sweeps: load(j) issues the
-// gathers of the four friends, the next quad's indices are fetched while they
-// are in flight, then add(c) folds them in list order (the reference's order).
+// Walks the first m entries of particle i's list in quad layout, in two
+// phases per quad: load(j) issues the gathers of the four friends, the next
+// quad's indices are fetched while they are in flight, then add(c) folds them
+// in list order (the reference's order).
 template <class Load, class Add>
 __device__ __forceinline__ void walk_quads(uint32_t i, int m, const uint4* __restrict__ quads, Load&& load,
                                            Add&& add) {

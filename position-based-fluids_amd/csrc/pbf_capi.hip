@@ -325,10 +325,12 @@ int enqueue_step(pbf_ctx* ctx, uint32_t pause) {
         ctx->cells_hold_keys = true;
         ctx->order_sorted = true;
     }
-    HIP_TRY(ctx, pbf::launch_friends(ctx));
+    // the friends build runs the first computeScaling as well, each thread
+    // from the list it has just written
+    HIP_TRY(ctx, pbf::launch_friends(ctx, true));
     const int K = (int)ctx->iters;
     for (int it = 0; it < K; ++it) {
-        HIP_TRY(ctx, pbf::launch_lambda(ctx, it));
+        if (it > 0) HIP_TRY(ctx, pbf::launch_lambda(ctx, it));
         HIP_TRY(ctx, pbf::launch_delta(ctx, it, it == K - 1));
     }
     HIP_TRY(ctx, pbf::launch_viscosity(ctx));

@@ -164,6 +164,57 @@ __global__ __launch_bounds__(256) void k_gather_cells(const SceneConsts* __restr
     if (i == n - 1) cells[2 * (size_t)k + 1] = i;
 }
 
+// compute_scaling.cl:1-113 + pack_data.cl (lambda into x*.w, in place: no
+// thread of this kernel reads another particle's w).
+struct LambdaAcc {
+    float dens = 0.0f, gsum = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    // Branch-free: the contributions of an out-of-range pair, and of a list
+    // entry past the walk's end (valid false), are selected to +0 (no sum
+    // here is ever -0, so the result is bitwise that of skipping it, also
+    // when the friend's position is not finite) and no branch joins the
+    // loop, which keeps the next gathers in flight.
+    __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c, bool valid) {
+        const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const uint32_t in = lane_mask(valid && r2 < c.h2);
+        const PairLen L = pair_len(r2);   // r/|r| = r * (1/|r|); 0 for a coincident pair
+        const float hr = c.h - L.rl;
+        const float s = c.spiky * hr * hr;
+        const float t = s * L.inv;
+        const float ax = masked(rx * t, in), ay = masked(ry * t, in), az = masked(rz * t, in);
+        const float d = masked(c.h2 - r2, in);
+        dens += d * d * d;
+        gsum += ax * ax + ay * ay + az * az;
+        gx += ax;
+        gy += ay;
+        gz += az;
+    }
+};
+
+// computeScaling for particle i from its sweep list (packed circle counts
+// and quads): density, lambda, and lambda into x*.w in place (no thread of a
+// launch reads another particle's w).  TAPE also stores (sum of gradients,
+// denominator) for the adjoint.
+template <bool TAPE>
+__device__ __forceinline__ void scaling_of(uint32_t i, uint32_t packed, float4* __restrict__ xs,
+                                           const uint4* __restrict__ quads, float* __restrict__ rho,
+                                           float* __restrict__ lambda, float4* __restrict__ aux, const SceneConsts& c) {
+    const float4 p = xs[i];
+    LambdaAcc a;
+    walk_friends<3, 5, true>(
+        i, packed, quads, [&](uint32_t j) { return xs[j]; },
+        [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
+    const float dens = a.dens * c.poly6;
+    rho[i] = dens;
+    const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
+    const float C = (dens / c.rest_density) - 1.0f;
+    const float den = gsum / (c.rest_density * c.rest_density) + c.epsilon;
+    const float lam = -1.0f * C / den;
+    lambda[i] = lam;
+    reinterpret_cast<float*>(xs + i)[3] = lam;
+    if (TAPE) aux[i] = make_float4(a.gx, a.gy, a.gz, den);
+}
+
 // ---------------------------------------------------------------------------
 // buildFriendsList (build_friends_list.cl:23-103).  27 cells in x, y, z order,
 // ascending j; circle = min(int(a^3 * 5), 4), a = max(0, (r - 0.3h) / 0.7h);
@@ -369,11 +420,14 @@ __device__ __forceinline__ uint32_t cells_to_scan(const float4& p, int32_t cx, i
     return mask;
 }
 
-template <int BS, bool STRIDED>
+// LAMBDA0 also runs the first computeScaling of the substep for the
+// particle, from the list its thread has just written (scaling_of).
+template <int BS, bool LAMBDA0, bool STRIDED>
 __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __restrict__ scenes,
-                                                      const uint4* __restrict__ blocks, const float4* __restrict__ xs,
+                                                      const uint4* __restrict__ blocks, float4* __restrict__ xs,
                                                       const uint2* __restrict__ cells, uint32_t* __restrict__ nb_cnt,
                                                       uint32_t* __restrict__ nb_list, uint4* __restrict__ quads,
+                                                      float* __restrict__ rho, float* __restrict__ lambda,
                                                       uint32_t n, uint32_t fresh, uint32_t* err,
                                                       uint32_t ntiles) {
     __shared__ uint32_t s_stage[4 * kCircles * BS];
@@ -397,6 +451,7 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
                                                quads, w);
             w.finish(i, nb_cnt, list, quads, err);
             pack_friends(i, w.packed, list, quads);
+            if (LAMBDA0) scaling_of<false>(i, w.packed, xs, quads, rho, lambda, nullptr, c);
         }
         if (!STRIDED) break;   // one workgroup per tile: a single pass
     }
@@ -409,11 +464,12 @@ template <> struct FriendsBudget<128> { static constexpr int cells = 256, staged
 template <> struct FriendsBudget<256> { static constexpr int cells = 512, staged = 2560; };
 template <> struct FriendsBudget<512> { static constexpr int cells = 1024, staged = 4096; };
 
-template <int BS>
+template <int BS, bool LAMBDA0>
 __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks,
-                                               const float4* __restrict__ xs, const uint2* __restrict__ cells,
+                                               float4* __restrict__ xs, const uint2* __restrict__ cells,
                                                uint32_t* __restrict__ nb_cnt, uint32_t* __restrict__ nb_list,
-                                               uint4* __restrict__ quads, uint32_t n, uint32_t fresh, uint32_t* err) {
+                                               uint4* __restrict__ quads, float* __restrict__ rho,
+                                               float* __restrict__ lambda, uint32_t n, uint32_t fresh, uint32_t* err) {
     constexpr int kWaves = BS / 64;
     constexpr int kMaxCells = FriendsBudget<BS>::cells;
     constexpr int kMaxStaged = FriendsBudget<BS>::staged;
@@ -536,6 +592,7 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
             }
     w.finish(i, nb_cnt, list, quads, err);
     pack_friends(i, w.packed, list, quads);
+    if (LAMBDA0) scaling_of<false>(i, w.packed, xs, quads, rho, lambda, nullptr, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -550,33 +607,6 @@ struct Pair4 {
     float4 a, b;
 };
 
-// compute_scaling.cl:1-113 + pack_data.cl (lambda into x*.w, in place: no
-// thread of this kernel reads another particle's w).
-struct LambdaAcc {
-    float dens = 0.0f, gsum = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    // Branch-free: the contributions of an out-of-range pair, and of a list
-    // entry past the walk's end (valid false), are selected to +0 (no sum
-    // here is ever -0, so the result is bitwise that of skipping it, also
-    // when the friend's position is not finite) and no branch joins the
-    // loop, which keeps the next gathers in flight.
-    __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c, bool valid) {
-        const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const uint32_t in = lane_mask(valid && r2 < c.h2);
-        const PairLen L = pair_len(r2);   // r/|r| = r * (1/|r|); 0 for a coincident pair
-        const float hr = c.h - L.rl;
-        const float s = c.spiky * hr * hr;
-        const float t = s * L.inv;
-        const float ax = masked(rx * t, in), ay = masked(ry * t, in), az = masked(rz * t, in);
-        const float d = masked(c.h2 - r2, in);
-        dens += d * d * d;
-        gsum += ax * ax + ay * ay + az * az;
-        gx += ax;
-        gy += ay;
-        gz += az;
-    }
-};
-
 // TAPE also stores (sum of gradients, denominator) of every particle for the
 // adjoint (aux).
 template <int BS, bool TAPE, bool STRIDED>
@@ -589,22 +619,7 @@ __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ s
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
         const uint32_t i = be.y + threadIdx.x;
-        if (i < be.z) {
-            const float4 p = xs[i];
-            LambdaAcc a;
-            walk_friends<3, 5, true>(
-                i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
-                [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
-            const float dens = a.dens * c.poly6;
-            rho[i] = dens;
-            const float gsum = a.gsum + (a.gx * a.gx + a.gy * a.gy + a.gz * a.gz);
-            const float C = (dens / c.rest_density) - 1.0f;
-            const float den = gsum / (c.rest_density * c.rest_density) + c.epsilon;
-            const float lam = -1.0f * C / den;
-            lambda[i] = lam;
-            reinterpret_cast<float*>(xs + i)[3] = lam;
-            if (TAPE) aux[i] = make_float4(a.gx, a.gy, a.gz, den);
-        }
+        if (i < be.z) scaling_of<TAPE>(i, nb_cnt[i], xs, nb_list, rho, lambda, aux, c);
         if (!STRIDED) break;   // one workgroup per tile: a single pass
     }
 }
@@ -829,7 +844,7 @@ uint32_t resident_blocks(const pbf_ctx* ctx, K kernel, int bs) {
 hipError_t query_residency(pbf_ctx* ctx) {
     for (int l = 6; l <= 9; ++l) {
         PBF_DISPATCH_BS(l, {
-            ctx->resident[F_FRIENDS][l - 6] = resident_blocks(ctx, k_friends_direct<BS, true>, BS);
+            ctx->resident[F_FRIENDS][l - 6] = resident_blocks(ctx, k_friends_direct<BS, true, true>, BS);
             ctx->resident[F_LAMBDA][l - 6] = resident_blocks(ctx, k_lambda<BS, false, true>, BS);
             ctx->resident[F_DELTA][l - 6] = resident_blocks(ctx, k_delta<BS, false, false, true>, BS);
             ctx->resident[F_VISC][l - 6] = resident_blocks(ctx, k_viscosity<BS, true>, BS);
@@ -889,19 +904,29 @@ hipError_t launch_gather(pbf_ctx* ctx) {
         });                                                                                                    \
     } while (0)
 
-hipError_t launch_friends(pbf_ctx* ctx) {
+hipError_t launch_friends(pbf_ctx* ctx, bool lambda0) {
     track_begin(ctx, "buildFriendsList");
     const uint32_t fresh = ctx->cells_fresh ? 1u : 0u;
     const BlockTable& t = table_for(ctx, F_FRIENDS);
+    float4* xs = ctx->xs[ctx->xs_cur];
+    const uint2* cells = reinterpret_cast<const uint2*>(ctx->cells);
     if (ctx->friends_staged) {
-        PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS],
-                        hipLaunchKernelGGL(k_friends<BS>, dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d,
-                                           ctx->xs[ctx->xs_cur], reinterpret_cast<const uint2*>(ctx->cells),
-                                           ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n, fresh, ctx->err));
+        PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS], {
+            if (lambda0)
+                hipLaunchKernelGGL((k_friends<BS, true>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, xs,
+                                   cells, ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->rho, ctx->lambda, ctx->n,
+                                   fresh, ctx->err);
+            else
+                hipLaunchKernelGGL((k_friends<BS, false>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, xs,
+                                   cells, ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->rho, ctx->lambda, ctx->n,
+                                   fresh, ctx->err);
+        });
+    } else if (lambda0) {
+        PBF_LAUNCH_TILED(F_FRIENDS, k_friends_direct<BS COMMA true, ctx->sc_d, t.d, xs, cells, ctx->nb_cnt,
+                         ctx->nb_list, ctx->nb_quads, ctx->rho, ctx->lambda, ctx->n, fresh, ctx->err);
     } else {
-        PBF_LAUNCH_TILED(F_FRIENDS, k_friends_direct<BS, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur],
-                         reinterpret_cast<const uint2*>(ctx->cells), ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->n,
-                         fresh, ctx->err);
+        PBF_LAUNCH_TILED(F_FRIENDS, k_friends_direct<BS COMMA false, ctx->sc_d, t.d, xs, cells, ctx->nb_cnt,
+                         ctx->nb_list, ctx->nb_quads, ctx->rho, ctx->lambda, ctx->n, fresh, ctx->err);
     }
     track_end(ctx);
     return hipGetLastError();

@@ -161,7 +161,8 @@ hipError_t launch_predict_hash(pbf_ctx* ctx, bool reset_old);
 hipError_t launch_predict_only(pbf_ctx* ctx, uint32_t pause);
 hipError_t launch_sort(pbf_ctx* ctx);
 hipError_t launch_gather(pbf_ctx* ctx);
-hipError_t launch_friends(pbf_ctx* ctx);
+// lambda0: the friends build also runs the substep's first computeScaling
+hipError_t launch_friends(pbf_ctx* ctx, bool lambda0 = false);
 // fills ctx->resident from the kernels' occupancy on ctx->cu_count CUs
 hipError_t query_residency(pbf_ctx* ctx);
 // aux / fut: per-particle records for the adjoint (nullptr: plain step)

@@ -491,6 +491,26 @@ def test_tiles_per_workgroup_do_not_change_results(tiles):
             assert np.array_equal(outs[0][k].view(np.uint32), outs[1][k].view(np.uint32)), (tiles, k)
 
 
+def test_strided_tiles_under_graph_replay():
+    """Strided launches (4 tiles per workgroup: 64 workgroups over 256
+    tiles) captured in the substep graph and replayed give the eager
+    one-tile-per-workgroup results bit for bit."""
+    fams = pbf_amd._lib.FAMILIES
+    scene = pbf_amd.config("dam16k", seed=3)
+    outs = []
+    for tiles, graph in ((1, False), (4, True)):
+        sim = pbf_amd.Simulation(scene.params, scene.options)
+        sim.set_state(scene.x, scene.v)
+        sim.set_workgroup_sizes({f: 64 for f in fams})
+        sim.set_tiles_per_workgroup({f: tiles for f in fams})
+        sim.set_graph(graph)
+        sim.run(6)
+        outs.append(sim.get_state())
+        sim.close()
+    for k in ("x", "v", "xs", "omega"):
+        assert np.array_equal(outs[0][k].view(np.uint32), outs[1][k].view(np.uint32)), k
+
+
 def test_persistent_grid_strides_at_1m():
     """At 1M particles and 64-thread workgroups the persistent grid is
     smaller than the 16,384 tiles, so every workgroup walks several tiles of

@@ -196,6 +196,10 @@ void Simulation::CopyRenderBuffers(std::vector<float>* positions4, std::vector<f
           "CopyRenderBuffers");
 }
 
+void Simulation::CopyRenderBuffers(float* positions4, float* texture4) const {
+    Check(pbf_get_state(mCtx, positions4, nullptr, texture4, nullptr, nullptr, nullptr), "CopyRenderBuffers");
+}
+
 void Simulation::GetState(std::vector<float>* x4, std::vector<float>* v4) const {
     const size_t n = Params.particleCount;
     if (x4) x4->resize(4 * n);

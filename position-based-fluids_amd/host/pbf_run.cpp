@@ -6,7 +6,7 @@
 // frame.  Prints one line of state statistics per reported frame and one
 // line per reload.
 //
-//   pbf_run <scenario.par> <frames> [report_every] [ramp] [--watch] [--fps N] [--log]
+//   pbf_run <scenario.par> <frames> [report_every] [ramp] [--watch] [--fps N] [--log] [--render]
 //
 // ramp:    LoadForceMasks installs one collision quad, a ramp over the floor
 //          of the box (y = 0.4h at xMin up to 0.1 of the height at xMax),
@@ -15,11 +15,15 @@
 //          watches); without it the file is read once.
 // --fps N: pace the loop to N frames per second, as a renderer would.
 // --log:   print the device event log after each reported frame.
+// --render: stage the render buffers (CopyRenderBuffers, both forms) after
+//          each reported frame and print whether the VBO equals the state's
+//          positions and the two forms agree.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <list>
 #include <sstream>
@@ -66,14 +70,15 @@ const char* event_name(uint32_t code) {
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s <scenario.par> <frames> [report_every] [ramp] [--watch] [--fps N] [--log]\n",
+        std::fprintf(stderr,
+                     "usage: %s <scenario.par> <frames> [report_every] [ramp] [--watch] [--fps N] [--log] [--render]\n",
                      argv[0]);
         return 2;
     }
     const std::string path = argv[1];
     const int frames = std::atoi(argv[2]);
     int every = 10;
-    bool ramp = false, watch = false, log = false;
+    bool ramp = false, watch = false, log = false, render = false;
     double fps = 0.0;
     for (int a = 3; a < argc; ++a) {
         const std::string s = argv[a];
@@ -83,6 +88,8 @@ int main(int argc, char** argv) {
             watch = true;
         else if (s == "--log")
             log = true;
+        else if (s == "--render")
+            render = true;
         else if (s == "--fps" && a + 1 < argc)
             fps = std::atof(argv[++a]);
         else
@@ -127,6 +134,17 @@ int main(int argc, char** argv) {
                 }
                 std::printf("frame %d n %zu ke %.6e ymin %.5f ymax %.5f finite %d\n", frame, n, ke, ymin, ymax,
                             finite ? 1 : 0);
+                if (render) {
+                    std::vector<float> vbo, tex;
+                    sim.CopyRenderBuffers(&vbo, &tex);
+                    std::vector<float> vbo2(vbo.size(), -1.0f), tex2(tex.size(), 0.0f);
+                    sim.CopyRenderBuffers(vbo2.data(), tex2.data());
+                    const bool same_vbo = vbo.size() == x.size() &&
+                                          std::memcmp(vbo.data(), x.data(), 4 * x.size()) == 0;
+                    const bool same_forms = vbo == vbo2 && tex == tex2;
+                    std::printf("render frame %d vbo_is_x %d forms_agree %d rows %zu\n", frame, same_vbo ? 1 : 0,
+                                same_forms ? 1 : 0, tex.size() / (4 * 2048));
+                }
                 if (log) {
                     std::vector<pbf_log_msg> msgs;
                     uint64_t lost = 0;

@@ -27,6 +27,18 @@ def test_pbf_run_frame_loop():
     assert float(lines[-1][5]) > 0.0
 
 
+def test_pbf_run_render_staging():
+    # CopyRenderBuffers into vectors and into caller-owned memory: the VBO is
+    # the state's positions (w = speed), the texture has ceil(4096 / 2048) rows
+    r = subprocess.run([str(RUN), str(ROOT / "tests" / "golden" / "dam4k.par"), "20", "10", "--render"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("render")]
+    assert [ln[2] for ln in lines] == ["10", "20"]
+    for ln in lines:
+        assert ln[4] == "1" and ln[6] == "1" and ln[8] == "2", ln
+
+
 def test_pbf_run_with_a_ramp():
     # LoadForceMasks with one solid ramp quad: the run stays healthy and differs
     # from the plain run

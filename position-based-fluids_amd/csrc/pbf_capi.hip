@@ -117,7 +117,8 @@ void free_all(pbf_ctx* ctx) {
 
 int validate(const pbf_params* p, pbf_ctx* ctx, uint32_t scene) {
     if (!p) return fail(ctx, PBF_E_INVALID, "params is NULL");
-    if (!(p->h > 0.0f) || !isfinite(p->h)) return fail(ctx, PBF_E_INVALID, "scene %u: h must be positive and finite", scene);
+    if (!(p->h >= 0x1p-20f) || !isfinite(p->h))
+        return fail(ctx, PBF_E_INVALID, "scene %u: h must be finite and at least 2^-20", scene);
     if (!(p->timeStep > 0.0f)) return fail(ctx, PBF_E_INVALID, "scene %u: timeStep must be positive", scene);
     if (p->simIterations < 1 || p->simIterations > 64)
         return fail(ctx, PBF_E_INVALID, "scene %u: simIterations must be in [1, 64] (got %u)", scene, p->simIterations);

@@ -152,6 +152,18 @@ __device__ __forceinline__ PairLen pair_len(float r2) {
 }
 __device__ __forceinline__ float norm_of_sq(float s2) { return s2 >= kTinySq ? sqrt_normal(s2) : 0.0f; }
 
+// The forward sweeps' form: |r| is not selected, only 1/|r|.  Below 2^-96
+// the estimate is below 2^-47, and the sweeps use |r| only in h - |r|, which
+// rounds to h there for every h >= 2^-20 (the smallest h pbf_create
+// accepts), as with |r| = 0; a NaN r^2 gives a NaN length, whose pair the
+// sweeps select out.
+__device__ __forceinline__ PairLen pair_len_sweep(float r2) {
+    const bool ok = r2 >= kTinySq;
+    const float rl = sqrt_normal(r2);
+    const float inv = rcp_normal(rl);
+    return {rl, ok ? inv : 0.0f};
+}
+
 // Selection of a pair's contributions as arithmetic: lane_mask(keep) is all
 // ones when keep and zero otherwise, as a value the compiler cannot see
 // through (an empty asm statement), and masked(x, m) is x or +0 (also for a

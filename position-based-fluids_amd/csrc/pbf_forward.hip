@@ -177,7 +177,7 @@ struct LambdaAcc {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         const uint32_t in = lane_mask(valid && r2 < c.h2);
-        const PairLen L = pair_len(r2);   // r/|r| = r * (1/|r|); 0 for a coincident pair
+        const PairLen L = pair_len_sweep(r2);   // r/|r| = r * (1/|r|); 0 for a coincident pair
         const float hr = c.h - L.rl;
         const float s = c.spiky * hr * hr;
         const float t = s * L.inv;
@@ -632,7 +632,7 @@ struct DeltaAcc {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         const uint32_t in = lane_mask(valid && r2 < c.h2);   // selected to +0 otherwise (as in LambdaAcc)
-        const PairLen L = pair_len(r2);
+        const PairLen L = pair_len_sweep(r2);
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
@@ -708,7 +708,7 @@ struct ViscAcc {
         const uint32_t in = lane_mask(valid && r2 < c.h2 && fabsf(pj.w) > 1e-8f);
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float d = c.h2 - r2;
-        const PairLen L = pair_len(r2);
+        const PairLen L = pair_len_sweep(r2);
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
@@ -763,7 +763,7 @@ struct VortAcc {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         const uint32_t in = lane_mask(valid && r2 < c.h2 && fabsf(pj.w) > 1e-8f);
-        const PairLen L = pair_len(r2);
+        const PairLen L = pair_len_sweep(r2);
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;

@@ -61,7 +61,10 @@ int pbf_load_parameters(const char *text, pbf_params *p);
 
 /* Simulation::Simulation + InitBuffers + InitCells + InitKernels
  * (src/Simulation.cpp:56-63,206-255,129-204): allocates every device
- * buffer for params->particleCount particles. opt may be NULL. */
+ * buffer for params->particleCount particles. opt may be NULL.
+ * PBF_E_INVALID unless h is finite and >= 2^-20, timeStep > 0,
+ * simIterations in [1, 64], gridBufSize in [1, 2^30], restDensity != 0 and
+ * friendsCircles / particlesPerCircle are 5 / 50 (pbf_params_finalize). */
 int pbf_create(const pbf_params *params, const pbf_options *opt, pbf_ctx **out);
 
 /* A batch of independent scenes stepped together: params[s] and opts[s]

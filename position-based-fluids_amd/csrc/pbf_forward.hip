@@ -283,7 +283,10 @@ struct FriendWriter {
     __device__ __forceinline__ void take(uint32_t i, uint32_t j, int circ, uint4* __restrict__ list,
                                          uint4* __restrict__ sweep) {
         if (circ < 0 || j == i) return;
-        const uint32_t f = (packed >> (6 * circ)) & 63u;
+        // circ is 0..4 here: 24-bit multiplies run at the full VALU rate,
+        // 32-bit ones at a quarter of it
+        const uint32_t sh = __umul24((uint32_t)circ, 6u);
+        const uint32_t f = (packed >> sh) & 63u;
         if (f >= (uint32_t)kPerCircle) {
             ++dropped;
             return;
@@ -292,14 +295,16 @@ struct FriendWriter {
         col[(f & 3u) * BS] = j;
         if ((f & 3u) == 3u) {
             // quad f / 4 of the circle's segment, addressed without a branch on
-            // the circle (circle 0 in the sweep quads, the others in the circle
-            // quads)
-            const uint32_t c0 = lane_mask(circ == 0);
-            const uint32_t qo = (((uint32_t)(circ - 1) * kSegQuads) & ~c0) + (f >> 2);
-            uint4* base = c0 ? qbase0 : qbase1;
-            base[(size_t)qo * 64u] = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
+            // the circle: circle 0 in the sweep quads, circle c > 0 at quad
+            // (c - 1) * 13 of the circle quads, read from 6-bit fields at the
+            // same shift as the counts.  Quads of one particle lie 64 apart,
+            // 1 KiB, so the offset (at most 64 quads) fits 32 bits.
+            constexpr uint32_t kSegBase = (1u * kSegQuads << 12) | (2u * kSegQuads << 18) | (3u * kSegQuads << 24);
+            const uint32_t qo = ((kSegBase >> sh) & 63u) + (f >> 2);
+            char* base = reinterpret_cast<char*>(circ == 0 ? qbase0 : qbase1);
+            *reinterpret_cast<uint4*>(base + (qo << 10)) = make_uint4(col[0], col[BS], col[2 * BS], col[3 * BS]);
         }
-        packed += 1u << (6 * circ);
+        packed += 1u << sh;
     }
     __device__ __forceinline__ void finish(uint32_t i, uint32_t* __restrict__ nb_cnt, uint4* __restrict__ list,
                                            uint4* __restrict__ sweep, uint32_t* err) {
@@ -367,7 +372,8 @@ __device__ __forceinline__ void scan_bucket_global(const float4& p, uint32_t i, 
     const uint32_t e = min(b.y, n - 1);
     uint32_t j = b.x;
     for (; j + 3 <= e; j += 4) {
-        const float4 q0 = xs[j], q1 = xs[j + 1], q2 = xs[j + 2], q3 = xs[j + 3];
+        const float4* __restrict__ g = xs + j;   // one address, the other three at immediate offsets
+        const float4 q0 = g[0], q1 = g[1], q2 = g[2], q3 = g[3];
         const int c0 = friend_circle(p, q0, c), c1 = friend_circle(p, q1, c);
         const int c2 = friend_circle(p, q2, c), c3 = friend_circle(p, q3, c);
         w.take(i, j, c0, nb_list, sweep); w.take(i, j + 1, c1, nb_list, sweep);
@@ -443,12 +449,12 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
             w.stage = s_stage + threadIdx.x;
             w.bind(i, list, quads);
             const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
+            const NeighbourHash nh(cx, cy, cz);
             for (int dx = -1; dx <= 1; ++dx)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dz = -1; dz <= 1; ++dz)
                         if (scan & (1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1))))
-                            scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list,
-                                               quads, w);
+                            scan_bucket_global(p, i, cells[nh(dx, dy, dz, c)], n, xs, c, list, quads, w);
             w.finish(i, nb_cnt, list, quads, err);
             pack_friends(i, w.packed, list, quads);
             if (LAMBDA0) scaling_of<false>(i, w.packed, xs, quads, rho, lambda, nullptr, c);
@@ -568,6 +574,7 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
     w.stage = s_stage + tid;
     w.bind(i, list, quads);
     const uint32_t scan = cells_to_scan(p, cx, cy, cz, c, fresh);
+    const NeighbourHash nh(cx, cy, cz);
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
@@ -586,8 +593,7 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
                     }
                     for (; k < cnt; ++k) w.take(i, gs + k, friend_circle(p, s_pos[off + k], c), list, quads);
                 } else {
-                    scan_bucket_global(p, i, cells[cell_hash(cx + dx, cy + dy, cz + dz, c)], n, xs, c, list, quads,
-                                       w);
+                    scan_bucket_global(p, i, cells[nh(dx, dy, dz, c)], n, xs, c, list, quads, w);
                 }
             }
     w.finish(i, nb_cnt, list, quads, err);

@@ -379,7 +379,18 @@ __device__ __forceinline__ void scan_bucket_global(const float4& p, uint32_t i, 
         w.take(i, j, c0, nb_list, sweep); w.take(i, j + 1, c1, nb_list, sweep);
         w.take(i, j + 2, c2, nb_list, sweep); w.take(i, j + 3, c3, nb_list, sweep);
     }
-    for (; j <= e; ++j) w.take(i, j, friend_circle(p, xs[j], c), nb_list, sweep);
+    // the last 1..3 candidates as one group: the loads clamped to e stay in
+    // the bucket and the slots past e are marked out of range, so the wave
+    // waits for one round of loads instead of up to three in turn
+    if (j <= e) {
+        const uint32_t r = e - j;   // 0..2
+        const float4* __restrict__ g = xs + j;
+        const float4 q0 = g[0], q1 = g[min(1u, r)], q2 = g[min(2u, r)];
+        const int c0 = friend_circle(p, q0, c);
+        const int c1 = (int)((uint32_t)friend_circle(p, q1, c) | ~lane_mask(r >= 1u));
+        const int c2 = (int)((uint32_t)friend_circle(p, q2, c) | ~lane_mask(r >= 2u));
+        w.take(i, j, c0, nb_list, sweep); w.take(i, j + 1, c1, nb_list, sweep); w.take(i, j + 2, c2, nb_list, sweep);
+    }
 }
 
 // Neighbour cells that cannot hold a friend.  Cell k covers [k h, (k+1) h)

@@ -39,18 +39,42 @@ STAGE_BYTES = {
 
 # HBM bytes per launch measured with rocprofv3 PMC counters on the
 # 1,048,576-particle dam break (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a
-# plain 256 MiB copy gives the two counters), averaged over the workgroup
-# sizes and, for computeDelta, over the K launches of a substep (the last one
-# also writes v and x); profiles/pmc_traffic_dam1m.md holds the counts and
-# commands.
-PMC_TRAFFIC_BYTES_PER_LAUNCH = {
-    "predictPositions": 43.9e6, "onesweep": 17.8e6, "sortParticles": 97.5e6,
-    "computeScaling": 185.3e6, "computeDelta": 190.0e6, "applyViscosity": 204.1e6, "applyVorticity": 202.5e6,
+# plain 256 MiB copy gives the two counters), averaged over the K launches of
+# a substep for computeDelta (the last one also writes v and x);
+# profiles/pmc_traffic_dam1m.md holds the counts and commands.  They belong
+# to the kernel sources whose hash is stamped here (KERNEL_SOURCES): with any
+# other source the line reports traffic as null.
+PMC_TRAFFIC = {
+    "source_sha16": "",
+    "bytes_per_launch": {},
 }
+KERNEL_SOURCES = ("pbf_forward.hip", "pbf_device.h", "pbf_sort.hip", "pbf_internal.h")
 
 
-def kernel_bytes_per_particle(name: str, iters: int, passes: int) -> float | None:
+def kernel_source_sha16() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        h.update((ROOT / "position-based-fluids_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def pmc_traffic(family_name: str, n: int) -> float | None:
+    """Measured HBM bytes per launch of a kernel family at 1,048,576
+    particles, or None when the table was measured on other kernel sources."""
+    if n != 1048576 or PMC_TRAFFIC["source_sha16"] != kernel_source_sha16():
+        return None
+    return PMC_TRAFFIC["bytes_per_launch"].get(family_name)
+
+
+def kernel_bytes_per_particle(name: str, iters: int, passes: int, friends: bool = False) -> float | None:
+    """Algorithmic bytes per particle of one launch.  The friends build has a
+    figure only for the first computeScaling it runs (its lists are index
+    structure, which the algorithmic figure excludes); friends=False leaves
+    it out, as the roofline kernel is chosen among the streaming stages."""
     base = name.rsplit("_", 1)[0] if name.rsplit("_", 1)[-1].isdigit() else name
+    if base == "buildFriendsList":
+        return STAGE_BYTES["lambda"] if friends else None
     if base == "predictPositions":
         return STAGE_BYTES["predict_hash"]
     if base == "onesweep":
@@ -104,25 +128,54 @@ def _oracle_rate(scene, max_steps: int, budget_s: float) -> tuple[float, int, fl
     return scene.n * steps / el, steps, el
 
 
-def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
+def cpu_baseline(scene, budget_s: float = 15.0, runs: int = 3) -> dict:
+    """The oracle on this host's cores: the median of `runs` runs of the
+    1M-particle dam break (substeps 2..21 from the initial state) and of the
+    16,384-particle dam break over its full 200 substeps."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pbf_oracle
     import pbf_amd
-    rate, steps, el = _oracle_rate(scene, 20, budget_s)
-    # the 16,384-particle dam break over its full 200 substeps
+    big = sorted((_oracle_rate(scene, 20, budget_s) for _ in range(runs)), key=lambda r: r[0])
+    rate, steps, el = big[len(big) // 2]
     small = pbf_amd.config("dam16k", seed=0)
-    rate16k, steps16k, el16k = _oracle_rate(small, 200, budget_s)
+    sm = sorted((_oracle_rate(small, 200, budget_s) for _ in range(runs)), key=lambda r: r[0])
+    rate16k, steps16k, el16k = sm[len(sm) // 2]
     return {
         "value": rate,
         "unit": "particle-substeps/s",
         "cores": int(pbf_oracle.lib().pbfo_threads()),
         "kind": "port",
-        "sample": f"dam1m substeps 2..{steps + 1} from the initial state ({steps} timed, {el:.1f} s), "
-                  f"C/OpenMP restatement of the reference kernels",
+        "sample": f"dam1m substeps 2..{steps + 1} from the initial state ({steps} timed, {el:.1f} s), median of "
+                  f"{runs} runs, C/OpenMP restatement of the reference kernels",
+        "runs": [r[0] for r in big],
         "host": {"nproc": os.cpu_count(), "model": _cpu_model()},
         "dam16k": {"value": rate16k, "steps": steps16k, "seconds": round(el16k, 2),
-                   "ms_per_substep": round(el16k / steps16k * 1e3, 3)},
+                   "ms_per_substep": round(el16k / steps16k * 1e3, 3), "runs": [r[0] for r in sm]},
     }
+
+
+def full_run(scene, sizes: dict | None) -> dict:
+    """All 500 substeps of the workload from its initial state, replayed as
+    HIP graphs (the default launch shapes, or the tuned sizes), with the
+    error flags and dropped friends they raise: the late substeps where
+    circles reach the 50-friend cap are timed too."""
+    import pbf_amd
+    sim = pbf_amd.Simulation(scene.params, scene.options)
+    sim.set_state(scene.x, scene.v)
+    if sizes:
+        sim.set_workgroup_sizes(sizes)
+    sim.set_graph(True)
+    sim.run(2)                      # captures the graphs of every buffer rotation
+    sim.set_state(scene.x, scene.v)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(500)
+    sim.sync()
+    el = time.perf_counter() - t0
+    flags, dropped = sim.error_flags()
+    sim.close()
+    return {"substeps": 500, "seconds": el, "particle_substeps_per_s": scene.n * 500 / el,
+            "ms_per_substep": el * 1e3 / 500, "error_flags": flags, "dropped_friends": dropped}
 
 
 def batched_gain(steps: int, warmup: int, seed: int, tune: bool) -> dict:
@@ -364,6 +417,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-full-run", action="store_true", help="skip the 500-substep run from the initial state")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default workgroup sizes")
     ap.add_argument("--batched-steps", type=int, default=30)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint and batched-differentiation lines")
@@ -436,15 +490,17 @@ def main() -> int:
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     step_ms = {k: round(v / prof_steps, 4) for k, v in sorted(fam_ms.items(), key=lambda kv: -kv[1])}
 
-    # the same figures for every family with a byte model, with the HBM rate
-    # that the PMC traffic gives at the measured launch time
+    # the same figures for every family with a byte model (the friends build
+    # with its first computeScaling included), with the HBM rate that the PMC
+    # traffic gives at the measured launch time
     by_kernel = {}
-    for fam_name in measured:
+    for fam_name in [f for f in fam_ms if kernel_bytes_per_particle(f, iters, passes, friends=True) is not None]:
         ls = [(name, t) for name, t in timings.items() if family(name) == fam_name]
         calls = sum(t["calls"] for _, t in ls)
         ms = sum(t["sum_ms"] for _, t in ls) / max(calls, 1)
-        alg = sum(kernel_bytes_per_particle(name, iters, passes) * n * t["calls"] for name, t in ls) / max(calls, 1)
-        traffic = PMC_TRAFFIC_BYTES_PER_LAUNCH.get(fam_name) if n == 1048576 else None
+        alg = sum(kernel_bytes_per_particle(name, iters, passes, friends=True) * n * t["calls"]
+                  for name, t in ls) / max(calls, 1)
+        traffic = pmc_traffic(fam_name, n)
         by_kernel[fam_name] = {
             "avg_launch_ms": round(ms, 5),
             "algorithmic_bytes_per_launch": alg,
@@ -485,7 +541,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": PMC_TRAFFIC_BYTES_PER_LAUNCH.get(dom) if n == 1048576 else None,
+            "traffic": pmc_traffic(dom, n),
+            "traffic_source_sha16": PMC_TRAFFIC["source_sha16"] or None,
+            "kernel_source_sha16": kernel_source_sha16(),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
             "top_kernel_overall": top,
@@ -495,6 +553,10 @@ def main() -> int:
         "error_flags": flags,
         "dropped_friends": dropped,
     }
+    if not args.no_full_run:
+        out["full_run"] = full_run(scene, sizes if not args.no_autotune else None)
+        _progress(f"full run: {out['full_run']['particle_substeps_per_s']:.4g} particle-substeps/s over 500 "
+                  f"substeps, {out['full_run']['dropped_friends']} friends dropped")
     if not args.no_batched:
         out["batched"] = batched_gain(args.batched_steps, 5, args.seed, not args.no_autotune)
         out["batched_gain"] = out["batched"]["gain"]

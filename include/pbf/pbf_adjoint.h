@@ -70,11 +70,37 @@ int pbf_adjoint_forward(pbf_adjoint* a, const float* x0, const float* v0, float*
  * {dL/dviscosity, dL/dvorticity, dL/dg0, dL/dg1, dL/dg2} (5 floats per scene,
  * device memory).  Requires a preceding pbf_adjoint_forward; may be called
  * again for another loss.  Leaves ctx's particles at an intermediate
- * substep.  Fails with PBF_E_STATE if a substep dropped friends (the
- * 50-per-circle cap) or visited a hash bucket twice, which the exact
- * transposition of the friend lists does not cover. */
+ * substep.  Substeps that dropped friends at the 50-per-circle cap or visited
+ * a hash bucket twice are differentiated too (their friend lists are
+ * transposed by a search of the neighbour cells).  Fails with PBF_E_STATE
+ * only when a particle is listed by more than 252 others (its transposed
+ * list does not fit its slot: a dense cluster at the cap), or when a listed
+ * friend lies outside the particle's 27 neighbour cells (possible only at
+ * the rounding of a cell boundary), rather than differentiate a different
+ * function. */
 int pbf_adjoint_backward(pbf_adjoint* a, const float* gxT, const float* gvT, float* gx0, float* gv0,
                          float* gcoef);
+
+/* Scenes whose gradients from the last pbf_adjoint_backward exceed binary32.
+ * The sensitivities of a long rollout can grow geometrically with the
+ * horizon; the backward keeps each scene's adjoint state as true * 2^e (exact
+ * power-of-two rescaling), so it stays finite, but a gradient whose true
+ * value is beyond binary32 is returned as +-inf.  *scenes receives how many
+ * scenes had one; per_scene (may be NULL; one word per scene) receives bit 0
+ * when a coefficient gradient overflowed and bit 1 when a dL/dx0 or dL/dv0
+ * component did.  No counterpart in the reference.  PBF_E_STATE before a
+ * backward. */
+int pbf_adjoint_overflow(pbf_adjoint* a, uint32_t* scenes, uint32_t* per_scene);
+
+/* The gradients of the last pbf_adjoint_backward as mantissas with one binary
+ * exponent per scene, finite where their true values exceed binary32: for
+ * the particles and coefficients of scene s, dL/dx0 = gx0[i] * 2^expo[s],
+ * dL/dv0 = gv0[i] * 2^expo[s], dL/dcoef = gcoef[5s + k] * 2^expo[s].  gx0,
+ * gv0, gcoef are device arrays laid out as in pbf_adjoint_backward, expo a
+ * host array of one int32 per scene; any may be NULL.  The largest x0 or v0
+ * mantissa component of a scene is in [2^-17, 2^16) unless all are zero.  No counterpart in the reference.  PBF_E_STATE before a
+ * backward. */
+int pbf_adjoint_scaled_gradients(pbf_adjoint* a, float* gx0, float* gv0, float* gcoef, int32_t* expo);
 
 /* Milliseconds of the last forward and backward (host clock around the
  * device work), and of the replays inside the last backward. */

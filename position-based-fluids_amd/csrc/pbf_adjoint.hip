@@ -668,13 +668,15 @@ __global__ __launch_bounds__(kAdjThreads) void k_lambda_b(
 }
 
 // Per-scene sums of the per-tile partials (tiles of a scene are contiguous),
-// added to the running totals in true units, and the power-of-two rescale of
+// added to the running coefficient totals, and the power-of-two rescale of
 // the scene's adjoint state for the next substep.  Sensitivities of these
 // flows grow geometrically with the horizon; the state (dL/dx, dL/dv) is kept
 // as true * 2^expo[s] with its largest component in [2^-16, 2^16], so that
-// no intermediate product overflows binary32.  Scaling by a power of two is
-// exact: the results equal the unscaled arithmetic wherever that stays
-// finite.  One wave per scene.
+// no intermediate product overflows binary32.  The totals are kept in the
+// same units as the state (gparam = true * 2^expo[s]) and rescaled with it,
+// so they stay finite where their true value exceeds binary32.  Scaling by a
+// power of two is exact: the results equal the unscaled arithmetic wherever
+// that stays finite and normal.  One wave per scene.
 __global__ __launch_bounds__(64) void k_param_reduce(const uint2* __restrict__ tile_range,
                                                     const float* __restrict__ part, float* __restrict__ gparam,
                                                     float* __restrict__ scale, int* __restrict__ expo) {
@@ -687,7 +689,7 @@ __global__ __launch_bounds__(64) void k_param_reduce(const uint2* __restrict__ t
         for (uint32_t t = tr.x + lane; t < tr.y; t += 64) a += part[(size_t)t * kParts + k];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-        if (lane == 0) gparam[s * 5 + k] += ldexpf(a, -E);
+        if (lane == 0) gparam[s * 5 + k] += a;
     }
     float mx = 0.0f;
     for (uint32_t t = tr.x + lane; t < tr.y; t += 64) {
@@ -708,7 +710,30 @@ __global__ __launch_bounds__(64) void k_param_reduce(const uint2* __restrict__ t
         }
         scale[s] = ldexpf(1.0f, k);
         expo[s] = E + k;
+        for (int q = 0; q < 5; ++q) gparam[s * 5 + q] = ldexpf(gparam[s * 5 + q], k);
     }
+}
+
+// Coefficient gradients in true units, gparam * 2^-expo; a finite total whose
+// true value exceeds binary32 comes out as +-inf and sets bit 0 of the
+// scene's overflow word.
+__global__ __launch_bounds__(64) void k_coef_out(const float* __restrict__ gparam, const int* __restrict__ expo,
+                                                 float* __restrict__ gcoef, uint32_t* __restrict__ ovf, uint32_t S) {
+    const uint32_t q = blockIdx.x * 64 + threadIdx.x;
+    if (q >= 5 * S) return;
+    const uint32_t s = q / 5;
+    const float m = gparam[q];
+    const float t = ldexpf(m, -expo[s]);
+    gcoef[q] = t;
+    if (isfinite(m) && !isfinite(t)) atomicOr(ovf + s, 1u);
+}
+
+// The same totals as mantissa and exponent: gparam itself (the exponent,
+// -expo[s], is read on the host).
+__global__ __launch_bounds__(64) void k_coef_scaled(const float* __restrict__ gparam, float* __restrict__ gcoef,
+                                                    uint32_t S) {
+    const uint32_t q = blockIdx.x * 64 + threadIdx.x;
+    if (q < 5 * S) gcoef[q] = gparam[q];
 }
 
 __global__ __launch_bounds__(64) void k_scale_init(float* __restrict__ scale, int* __restrict__ expo, uint32_t S) {
@@ -722,17 +747,31 @@ __global__ __launch_bounds__(64) void k_scale_init(float* __restrict__ scale, in
 // Gradients in true units (identity order: the rollout's first substep).
 // The state is still in the units of the last substep: the rescale chosen
 // after it (scale = 2^k, already counted in expo) has not been applied.
+// SCALED writes the mantissas instead, state * scale: the true values are
+// those times 2^-expo.  A finite component whose true value exceeds binary32
+// comes out as +-inf and sets bit 1 of the scene's overflow word.
+template <bool SCALED>
 __global__ __launch_bounds__(kAdjThreads) void k_unscale(const uint4* __restrict__ blocks,
                                                         const float* __restrict__ scale, const int* __restrict__ expo,
                                                         const float4* __restrict__ gx, const float4* __restrict__ gv,
-                                                        float4* __restrict__ gx0, float4* __restrict__ gv0) {
+                                                        float4* __restrict__ gx0, float4* __restrict__ gv0,
+                                                        uint32_t* __restrict__ ovf) {
     const uint4 be = blocks[blockIdx.x];
     const uint32_t i = be.y + threadIdx.x;
     if (i >= be.z) return;
-    const int e = ilogbf(scale[be.x]) - expo[be.x];
+    const int e = SCALED ? ilogbf(scale[be.x]) : ilogbf(scale[be.x]) - expo[be.x];
     const float4 a = gx[i], b = gv[i];
-    if (gx0) gx0[i] = make_float4(ldexpf(a.x, e), ldexpf(a.y, e), ldexpf(a.z, e), 0.0f);
-    if (gv0) gv0[i] = make_float4(ldexpf(b.x, e), ldexpf(b.y, e), ldexpf(b.z, e), 0.0f);
+    const float4 ta = make_float4(ldexpf(a.x, e), ldexpf(a.y, e), ldexpf(a.z, e), 0.0f);
+    const float4 tb = make_float4(ldexpf(b.x, e), ldexpf(b.y, e), ldexpf(b.z, e), 0.0f);
+    if (gx0) gx0[i] = ta;
+    if (gv0) gv0[i] = tb;
+    if (!SCALED) {
+        const bool before = isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(b.x) && isfinite(b.y) &&
+                            isfinite(b.z);
+        const bool after = isfinite(ta.x) && isfinite(ta.y) && isfinite(ta.z) && isfinite(tb.x) && isfinite(tb.y) &&
+                           isfinite(tb.z);
+        if (before && !after) atomicOr(ovf + be.x, 2u);
+    }
 }
 
 // One recorded substep (device arrays).
@@ -769,13 +808,17 @@ struct pbf_adjoint {
     float4 *z = nullptr, *z2 = nullptr;
     float* zb = nullptr;
     float* part = nullptr;
-    float* gparam = nullptr;
+    float* gparam = nullptr;               // per scene: coefficient gradients, true * 2^expo
+    float* gout = nullptr;                 // per scene: coefficient gradients in true units
     float* scale = nullptr;                // per scene: rescale of the adjoint state for the next substep
     int* expo = nullptr;                   // per scene: state = true * 2^expo
     uint2* tile_range = nullptr;
     unsigned long long* notes = nullptr;   // C + 1 snapshots of the dropped-friends counter
     uint32_t* flags = nullptr;             // kFlagWords: dup / capacity flags, transposition counts
+    uint32_t* ovf = nullptr;               // per scene: bit 0 coefficient, bit 1 x0/v0 gradient beyond binary32
+    std::vector<uint32_t> overflow;        // ovf of the last backward (host copy)
     bool have_forward = false;
+    bool have_backward = false;            // gx, gv, gparam, expo hold the last backward's results
     int recorded = -1;                     // segment whose substeps the slots hold
     double fwd_ms = 0.0, bwd_ms = 0.0, replay_ms = 0.0;
 };
@@ -1068,11 +1111,13 @@ int pbf_adjoint_create(pbf_ctx* ctx, uint32_t substeps, uint32_t interval, pbf_a
     const pbf::BlockTable& bt = ctx->bt[2];
     ADJ_TRY(pbf::aalloc(a, &a->part, (size_t)bt.count * pbf::kParts));
     ADJ_TRY(pbf::aalloc(a, &a->gparam, (size_t)ctx->S * 5));
+    ADJ_TRY(pbf::aalloc(a, &a->gout, (size_t)ctx->S * 5));
     ADJ_TRY(pbf::aalloc(a, &a->scale, ctx->S));
     ADJ_TRY(pbf::aalloc(a, &a->expo, ctx->S));
     ADJ_TRY(pbf::aalloc(a, &a->tile_range, ctx->S));
     ADJ_TRY(pbf::aalloc(a, &a->notes, (size_t)a->C + 1));
     ADJ_TRY(pbf::aalloc(a, &a->flags, pbf::kFlagWords));
+    ADJ_TRY(pbf::aalloc(a, &a->ovf, ctx->S));
     // tiles of the 256-particle block table per scene
     std::vector<uint2> tr(ctx->S);
     uint32_t tile = 0;
@@ -1101,6 +1146,7 @@ int pbf_adjoint_forward(pbf_adjoint* a, const float* x0, const float* v0, float*
     const double t0 = pbf::now_ms();
     const size_t n = ctx->n;
     a->have_forward = false;
+    a->have_backward = false;
     if (n) {
         HIP_TRY(ctx, hipMemcpyAsync(ctx->x, x0, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(ctx->v, v0, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1162,7 +1208,9 @@ int pbf_adjoint_backward(pbf_adjoint* a, const float* gxT, const float* gvT, flo
                        (size_t)ctx->S * 5);
     hipLaunchKernelGGL(pbf::k_scale_init, dim3((ctx->S + 63) / 64), dim3(64), 0, ctx->stream, a->scale, a->expo,
                        ctx->S);
+    HIP_TRY(ctx, hipMemsetAsync(a->ovf, 0, 4 * (size_t)ctx->S, ctx->stream));
     HIP_TRY(ctx, hipGetLastError());
+    a->have_backward = false;
     for (int g = (int)a->segs - 1; g >= 0; --g) {
         if (a->recorded != g) {
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -1180,17 +1228,61 @@ int pbf_adjoint_backward(pbf_adjoint* a, const float* gxT, const float* gvT, flo
             if (rc) return rc;
         }
     }
-    if (gx0 || gv0) {
+    {
+        // true units, flagging what exceeds binary32 (the x0/v0 check runs
+        // even when the caller takes neither)
         const pbf::BlockTable& bt = ctx->bt[2];
-        hipLaunchKernelGGL(pbf::k_unscale, dim3(bt.count), dim3(pbf::kAdjThreads), 0, ctx->stream, bt.d, a->scale,
-                           a->expo, a->gx, a->gv, reinterpret_cast<float4*>(gx0), reinterpret_cast<float4*>(gv0));
+        hipLaunchKernelGGL(pbf::k_unscale<false>, dim3(bt.count), dim3(pbf::kAdjThreads), 0, ctx->stream, bt.d,
+                           a->scale, a->expo, a->gx, a->gv, reinterpret_cast<float4*>(gx0),
+                           reinterpret_cast<float4*>(gv0), a->ovf);
+        hipLaunchKernelGGL(pbf::k_coef_out, dim3((5 * ctx->S + 63) / 64), dim3(64), 0, ctx->stream, a->gparam,
+                           a->expo, a->gout, a->ovf, ctx->S);
         HIP_TRY(ctx, hipGetLastError());
     }
     if (gcoef)
-        HIP_TRY(ctx, hipMemcpyAsync(gcoef, a->gparam, 4 * 5 * (size_t)ctx->S, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(gcoef, a->gout, 4 * 5 * (size_t)ctx->S, hipMemcpyDeviceToDevice, ctx->stream));
+    a->overflow.assign(ctx->S, 0u);
+    HIP_TRY(ctx, hipMemcpyAsync(a->overflow.data(), a->ovf, 4 * (size_t)ctx->S, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    a->have_backward = true;
     a->bwd_ms = pbf::now_ms() - t0;
     a->replay_ms = replay;
+    return PBF_OK;
+}
+
+int pbf_adjoint_overflow(pbf_adjoint* a, uint32_t* scenes, uint32_t* per_scene) {
+    if (!a || !scenes) return PBF_E_INVALID;
+    if (!a->have_backward) return pbf::fail(a->ctx, PBF_E_STATE, "pbf_adjoint_overflow before pbf_adjoint_backward");
+    uint32_t k = 0;
+    for (size_t s = 0; s < a->overflow.size(); ++s) {
+        if (a->overflow[s]) ++k;
+        if (per_scene) per_scene[s] = a->overflow[s];
+    }
+    *scenes = k;
+    return PBF_OK;
+}
+
+int pbf_adjoint_scaled_gradients(pbf_adjoint* a, float* gx0, float* gv0, float* gcoef, int32_t* expo) {
+    if (!a) return PBF_E_INVALID;
+    pbf_ctx* ctx = a->ctx;
+    if (!a->have_backward) return pbf::fail(ctx, PBF_E_STATE, "pbf_adjoint_scaled_gradients before pbf_adjoint_backward");
+    if (ctx->n == 0) return PBF_OK;
+    if (gx0 || gv0) {
+        const pbf::BlockTable& bt = ctx->bt[2];
+        hipLaunchKernelGGL(pbf::k_unscale<true>, dim3(bt.count), dim3(pbf::kAdjThreads), 0, ctx->stream, bt.d,
+                           a->scale, a->expo, a->gx, a->gv, reinterpret_cast<float4*>(gx0),
+                           reinterpret_cast<float4*>(gv0), a->ovf);
+    }
+    if (gcoef)
+        hipLaunchKernelGGL(pbf::k_coef_scaled, dim3((5 * ctx->S + 63) / 64), dim3(64), 0, ctx->stream, a->gparam,
+                           gcoef, ctx->S);
+    HIP_TRY(ctx, hipGetLastError());
+    if (expo) {
+        HIP_TRY(ctx, hipMemcpyAsync(expo, a->expo, 4 * (size_t)ctx->S, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (expo)
+        for (uint32_t s = 0; s < ctx->S; ++s) expo[s] = -expo[s];
     return PBF_OK;
 }
 

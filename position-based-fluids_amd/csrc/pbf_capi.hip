@@ -17,7 +17,9 @@
 #include "pbf_internal.h"
 
 namespace {
-std::string g_create_error;
+// message of the last failure with no context (pbf_create, pbf_run_noalloc);
+// a fixed buffer, so that recording it allocates nothing
+char g_create_error[512];
 }  // namespace
 
 namespace pbf {
@@ -30,7 +32,7 @@ int fail(pbf_ctx* ctx, int code, const char* fmt, ...) {
     if (ctx)
         ctx->last_error = buf;
     else
-        g_create_error = buf;
+        snprintf(g_create_error, sizeof g_create_error, "%s", buf);
     return code;
 }
 }  // namespace pbf
@@ -115,6 +117,10 @@ void free_all(pbf_ctx* ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
+}  // namespace
+
+namespace pbf {
+
 int validate(const pbf_params* p, pbf_ctx* ctx, uint32_t scene) {
     if (!p) return fail(ctx, PBF_E_INVALID, "params is NULL");
     if (!(p->h >= 0x1p-20f) || !isfinite(p->h))
@@ -169,6 +175,13 @@ pbf::SceneConsts make_consts(const pbf_params& P, const pbf_options& o, uint32_t
     return c;
 }
 
+}  // namespace pbf
+
+namespace {
+
+using pbf::make_consts;
+using pbf::validate;
+
 // Block table: one {scene, first, end, 0} per workgroup of `per` particles.
 std::vector<uint4> block_table(const pbf_ctx* ctx, uint32_t per) {
     std::vector<uint4> t;
@@ -216,7 +229,7 @@ int create_impl(const pbf_params* params, const pbf_options* opts, uint32_t S, p
     }
 
     auto bail = [&](int code) {
-        g_create_error = ctx->last_error;
+        snprintf(g_create_error, sizeof g_create_error, "%s", ctx->last_error.c_str());
         free_all(ctx);
         delete ctx;
         return code;
@@ -309,7 +322,9 @@ struct Rotation {
 
 Rotation rotation_of(const pbf_ctx* ctx) { return {ctx->x, ctx->x_alt, ctx->v, ctx->v_alt, ctx->xs_cur}; }
 
-int enqueue_step(pbf_ctx* ctx, uint32_t pause) {
+}  // namespace
+
+int pbf::enqueue_step(pbf_ctx* ctx, uint32_t pause) {
     ctx->pause_now = pause ? 1u : 0u;
     // Simulation.cpp:563-567: predict always; radixsort unless paused.  With
     // the particle order not sorted (no step yet, or new state) the step
@@ -338,6 +353,10 @@ int enqueue_step(pbf_ctx* ctx, uint32_t pause) {
     HIP_TRY(ctx, pbf::launch_vorticity(ctx));
     return PBF_OK;
 }
+
+namespace {
+
+using pbf::enqueue_step;
 
 // Graph key: everything that changes the launch sequence or its arguments.
 uint32_t graph_key(const pbf_ctx* ctx, uint32_t pause) {
@@ -558,7 +577,7 @@ void pbf_destroy(pbf_ctx* ctx) {
     delete ctx;
 }
 
-const char* pbf_last_error(const pbf_ctx* ctx) { return ctx ? ctx->last_error.c_str() : g_create_error.c_str(); }
+const char* pbf_last_error(const pbf_ctx* ctx) { return ctx ? ctx->last_error.c_str() : g_create_error; }
 
 int pbf_set_state(pbf_ctx* ctx, const float* x4, const float* v4, uint32_t n) {
     if (!ctx) return PBF_E_INVALID;

@@ -245,7 +245,7 @@ __device__ __host__ __forceinline__ size_t friend_quad(uint32_t i, int circle, i
 __device__ __host__ __forceinline__ size_t friend_slot(uint32_t i, int circle, int f) {   // in uint32 units
     return friend_quad(i, circle, f >> 2) * 4 + (size_t)(f & 3);
 }
-__device__ __host__ __forceinline__ size_t friend_slots(size_t n) { return ((n + 63) / 64) * 64 * kCircles * kSeg; }
+constexpr __device__ __host__ size_t friend_slots(size_t n) { return ((n + 63) / 64) * 64 * kCircles * kSeg; }
 
 // Sweep lists: the friends a sweep processes (the 0.6-threshold prefix, in
 // the reference's order) regrouped per block of 64 consecutive particles in

@@ -12,7 +12,7 @@
 
 namespace pbf {
 
-// Onesweep radix sort geometry: 256 threads, 8 keys per thread, 8-bit digits.
+// Onesweep radix sort geometry: 256 threads, 16 keys per thread, 8-bit digits.
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
@@ -174,6 +174,14 @@ hipError_t launch_vorticity(pbf_ctx* ctx);
 // Records a formatted error on ctx (or for the next pbf_create when ctx is
 // NULL) and returns code.
 int fail(pbf_ctx* ctx, int code, const char* fmt, ...);
+
+// Checks one scene's parameters (PBF_E_INVALID with a message on ctx).
+int validate(const pbf_params* p, pbf_ctx* ctx, uint32_t scene);
+// The device constants of a scene whose cells start at global cell id
+// cell_base and whose particles start at index first (no collision quads).
+SceneConsts make_consts(const pbf_params& P, const pbf_options& o, uint32_t cell_base, uint32_t first);
+// Enqueues one substep (Simulation::Step) on ctx->stream.
+int enqueue_step(pbf_ctx* ctx, uint32_t pause);
 
 // Profiling brackets (no-ops unless ctx->profiling).
 void track_begin(pbf_ctx* ctx, const char* name, int index = -1);

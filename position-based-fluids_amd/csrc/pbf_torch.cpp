@@ -47,7 +47,15 @@ public:
         const int S = pbf_scene_count(ctx);
         TORCH_CHECK(S > 0, "bad context");
         TORCH_CHECK(coef.dim() == 2 && coef.size(0) == S && coef.size(1) == 5, "coef must be (", S, ", 5)");
-        const int64_t n = x0.size(0);
+        // the rows must be the context's particles: the library reads and
+        // writes exactly that many float4 rows of every buffer it is given
+        pbf_dev_bufs db{};
+        check(pbf_device_buffers(ctx, &db), ctx);
+        const int64_t n = (int64_t)db.n;
+        TORCH_CHECK(x0.dim() >= 1 && x0.size(0) == n, "x0 must have ", n, " rows (the context's particles), got ",
+                    x0.sizes());
+        TORCH_CHECK(v0.dim() >= 1 && v0.size(0) == n, "v0 must have ", n, " rows (the context's particles), got ",
+                    v0.sizes());
         auto ch = coef.detach().to(torch::kCPU, torch::kFloat32).contiguous();
         const float* cp = ch.data_ptr<float>();
         for (int s = 0; s < S; ++s) check(pbf_set_scene_coefficients(ctx, (uint32_t)s, cp[5 * s], cp[5 * s + 1], cp + 5 * s + 2), ctx);

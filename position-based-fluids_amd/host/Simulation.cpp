@@ -20,7 +20,12 @@ void LoadParameters(const std::string& text) {
     if (rc < 0) throw std::runtime_error("LoadParameters: invalid input");
 }
 
-Simulation::Simulation() { pbf_default_options(&mOptions); }
+Simulation::Simulation() {
+    // the reference kernels apply no box walls, only the force plates
+    // (compute_delta.cl:163-182)
+    pbf_default_options(&mOptions);
+    mOptions.walls = 0;
+}
 
 Simulation::Simulation(const pbf_options& options) : mOptions(options) {}
 
@@ -104,25 +109,23 @@ bool DetectResourceChanges(std::list<std::pair<std::string, time_t>>& fileList) 
     return true;
 }
 
-// src/Simulation.cpp:71-101: a cube of ceil(N^(1/3)) particles per axis at
-// spacing h * setupSpacing, offset (1 - axis*d)/2 in x and z and 0.3 in y,
-// with the reference's index-to-axis assignment (y fastest, then x, then z).
+// The initial block of src/Simulation.cpp:71-101: a cube with
+// a = ceil(N^(1/3)) particles per side at spacing d = h * setupSpacing,
+// filled y first, then x, then z (particle i sits at lattice point
+// (floor(i/a) mod a, i mod a, floor(i/a^2) mod a)), centred on x = z = 1/2
+// and lifted to y = 0.3; v = 0.
 void Simulation::CreateParticles() {
     const uint32_t n = Params.particleCount;
     std::vector<float> x(4 * (size_t)n, 0.0f);
-    const int per_axis = (int)std::ceil(std::pow((double)n, 1.0 / 3.0));
+    const uint32_t a = (uint32_t)std::ceil(std::pow((double)n, 1.0 / 3.0));
     const float d = Params.h * Params.setupSpacing;
-    const float off_x = (1.0f - per_axis * d) / 2.0f;
-    const float off_y = 0.3f;
-    const float off_z = (1.0f - per_axis * d) / 2.0f;
+    const float side = (float)a * d;
+    const float corner_xz = (1.0f - side) / 2.0f;
     for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t ix = ((uint32_t)(i / std::pow(per_axis, 1)) % per_axis);
-        const uint32_t iy = ((uint32_t)(i / std::pow(per_axis, 0)) % per_axis);
-        const uint32_t iz = ((uint32_t)(i / std::pow(per_axis, 2)) % per_axis);
-        x[4 * (size_t)i + 0] = off_x + ix * d;
-        x[4 * (size_t)i + 1] = off_y + iy * d;
-        x[4 * (size_t)i + 2] = off_z + iz * d;
-        x[4 * (size_t)i + 3] = 0.0f;
+        const uint32_t col = i / a;
+        const float lattice[3] = {(float)(col % a), (float)(i % a), (float)((col / a) % a)};
+        const float corner[3] = {corner_xz, 0.3f, corner_xz};
+        for (int k = 0; k < 3; ++k) x[4 * (size_t)i + k] = corner[k] + lattice[k] * d;
     }
     Check(pbf_set_state(mCtx, x.data(), nullptr, n), "CreateParticles");
 }

@@ -49,7 +49,9 @@ struct PerfMon {
 class Simulation {
 public:
     // The reference takes the CL context and device; the HIP context is the
-    // current device of the calling thread.
+    // current device of the calling thread.  As in the reference kernels no
+    // box walls are applied (only the force plates of LoadForceMasks);
+    // options with walls = 1 add the six planes at x/y/zMin/Max.
     Simulation();
     explicit Simulation(const pbf_options& options);
     ~Simulation();

@@ -6,11 +6,24 @@
 // frame.  Prints one line of state statistics per reported frame and one
 // line per reload.
 //
-//   pbf_run <scenario.par> <frames> [report_every] [ramp] [--watch] [--fps N] [--log] [--render]
+//   pbf_run <scenario.par> <frames> [report_every] [ramp] [--walls] [--plates MASK] [--pause F,F,...]
+//           [--dump DIR] [--watch] [--fps N] [--log] [--render]
 //
 // ramp:    LoadForceMasks installs one collision quad, a ramp over the floor
 //          of the box (y = 0.4h at xMin up to 0.1 of the height at xMax),
 //          solid everywhere.
+// --walls: the six box planes at x/y/zMin/Max (the reference kernels have
+//          none; the drop-in Simulation leaves them off by default).
+// --plates MASK: LoadForceMasks with the reference scene's 14 force plates
+//          (edge offset 1.0) and MASK, a raw file of one byte per texel,
+//          512 texels per row (the red channel of the reference's mask).
+// --pause F,F,...: frames run with bPauseSim set.
+// --dump DIR: write raw little-endian arrays to DIR: init.x (float4 per
+//          particle after InitBuffers), and for every reported frame F
+//          F.vbo and F.tex (CopyRenderBuffers: positions with w = |v|, the
+//          2048-wide x* texture with w = density), F.ids (ping and pong
+//          buffer ids, uint32) and F.keys, F.perm, F.cells, F.friends
+//          (pbf_get_structure, friends in the reference layout).
 // --watch: reload the scenario whenever it changes (the reference always
 //          watches); without it the file is read once.
 // --fps N: pace the loop to N frames per second, as a renderer would.
@@ -19,6 +32,7 @@
 //          each reported frame and print whether the VBO equals the state's
 //          positions and the two forms agree.
 #include <algorithm>
+#include <set>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -56,6 +70,43 @@ void install_ramp(pbf_host::Simulation& sim) {
     sim.LoadForceMasks({1u}, 1, 1, {q}, 0.0f);
 }
 
+void write_raw(const std::string& path, const void* data, size_t bytes) {
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    const size_t w = bytes ? std::fwrite(data, 1, bytes, f) : 0;
+    std::fclose(f);
+    if (w != bytes) throw std::runtime_error("short write to " + path);
+}
+
+std::vector<uint32_t> read_mask(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::vector<char> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    if (bytes.empty() || bytes.size() % 512) throw std::runtime_error(path + ": not a whole number of 512-texel rows");
+    std::vector<uint32_t> texels(bytes.size());
+    for (size_t k = 0; k < bytes.size(); ++k) texels[k] = (uint8_t)bytes[k];
+    return texels;
+}
+
+void dump_frame(const std::string& dir, int frame, pbf_host::Simulation& sim) {
+    const std::string base = dir + "/" + std::to_string(frame);
+    std::vector<float> vbo, tex;
+    sim.CopyRenderBuffers(&vbo, &tex);
+    write_raw(base + ".vbo", vbo.data(), 4 * vbo.size());
+    write_raw(base + ".tex", tex.data(), 4 * tex.size());
+    const uint32_t ids[2] = {sim.mSharedPingBufferID, sim.mSharedPongBufferID};
+    write_raw(base + ".ids", ids, sizeof ids);
+    const size_t n = pbf_host::Params.particleCount;
+    std::vector<uint32_t> keys(n), perm(n), cells(2 * (size_t)pbf_host::Params.gridBufSize),
+        friends(n * pbf_host::Params.friendsCircles * (1 + pbf_host::Params.particlesPerCircle));
+    const int rc = pbf_get_structure(sim.context(), keys.data(), perm.data(), cells.data(), friends.data());
+    if (rc != PBF_OK) throw std::runtime_error(std::string("pbf_get_structure: ") + pbf_last_error(sim.context()));
+    write_raw(base + ".keys", keys.data(), 4 * n);
+    write_raw(base + ".perm", perm.data(), 4 * n);
+    write_raw(base + ".cells", cells.data(), 4 * cells.size());
+    write_raw(base + ".friends", friends.data(), 4 * friends.size());
+}
+
 const char* event_name(uint32_t code) {
     switch (code) {
         case PBF_LOG_FRIENDS_OVERFLOW: return "friends_overflow";
@@ -71,19 +122,32 @@ const char* event_name(uint32_t code) {
 int main(int argc, char** argv) {
     if (argc < 3) {
         std::fprintf(stderr,
-                     "usage: %s <scenario.par> <frames> [report_every] [ramp] [--watch] [--fps N] [--log] [--render]\n",
+                     "usage: %s <scenario.par> <frames> [report_every] [ramp] [--walls] [--plates MASK] "
+                     "[--pause F,F,...] [--dump DIR] [--watch] [--fps N] [--log] [--render]\n",
                      argv[0]);
         return 2;
     }
     const std::string path = argv[1];
     const int frames = std::atoi(argv[2]);
     int every = 10;
-    bool ramp = false, watch = false, log = false, render = false;
+    bool ramp = false, watch = false, log = false, render = false, walls = false;
+    std::string plates, dump;
+    std::set<int> paused;
     double fps = 0.0;
     for (int a = 3; a < argc; ++a) {
         const std::string s = argv[a];
         if (s == "ramp")
             ramp = true;
+        else if (s == "--walls")
+            walls = true;
+        else if (s == "--plates" && a + 1 < argc)
+            plates = argv[++a];
+        else if (s == "--dump" && a + 1 < argc)
+            dump = argv[++a];
+        else if (s == "--pause" && a + 1 < argc) {
+            std::stringstream list(argv[++a]);
+            for (std::string f; std::getline(list, f, ',');) paused.insert(std::atoi(f.c_str()));
+        }
         else if (s == "--watch")
             watch = true;
         else if (s == "--log")
@@ -96,7 +160,11 @@ int main(int argc, char** argv) {
             every = std::max(1, std::atoi(argv[a]));
     }
     try {
-        pbf_host::Simulation sim;
+        pbf_options opt;
+        pbf_default_options(&opt);
+        opt.walls = walls ? 1u : 0u;
+        pbf_host::Simulation sim(opt);
+        const std::vector<uint32_t> mask = plates.empty() ? std::vector<uint32_t>() : read_mask(plates);
         std::list<std::pair<std::string, time_t>> tracked = {{path, 0}};
         uint32_t prev_count = 0;
         bool kernels_ok = false;
@@ -111,14 +179,24 @@ int main(int argc, char** argv) {
                 if (reset) {
                     prev_count = pbf_host::Params.particleCount;
                     sim.InitBuffers();
+                    // the renderer's two position buffers (src/Runner.cpp:58-59)
+                    sim.mSharedPingBufferID = 1;
+                    sim.mSharedPongBufferID = 2;
+                    if (!dump.empty()) {
+                        std::vector<float> x0, v0;
+                        sim.GetState(&x0, &v0);
+                        write_raw(dump + "/init.x", x0.data(), 4 * x0.size());
+                    }
                 }
                 sim.InitCells();
                 if (ramp) install_ramp(sim);
+                if (!mask.empty()) sim.LoadForceMasks(mask, 512, (uint32_t)(mask.size() / 512));
                 kernels_ok = sim.InitKernels();
                 if (frame > 1) std::printf("reload frame %d reset %d ok %d\n", frame, reset ? 1 : 0, kernels_ok ? 1 : 0);
                 std::fflush(stdout);
             }
             if (!kernels_ok) continue;
+            sim.bPauseSim = paused.count(frame) != 0;
             for (uint32_t s = 0; s < pbf_host::Params.subSteps; ++s) sim.Step();
             if (frame % every == 0 || frame == frames) {
                 sim.GetState(&x, &v);
@@ -145,6 +223,7 @@ int main(int argc, char** argv) {
                     std::printf("render frame %d vbo_is_x %d forms_agree %d rows %zu\n", frame, same_vbo ? 1 : 0,
                                 same_forms ? 1 : 0, tex.size() / (4 * 2048));
                 }
+                if (!dump.empty()) dump_frame(dump, frame, sim);
                 if (log) {
                     std::vector<pbf_log_msg> msgs;
                     uint64_t lost = 0;

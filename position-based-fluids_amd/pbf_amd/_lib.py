@@ -210,6 +210,8 @@ _SIGNATURES = {
     "pbf_adjoint_backward": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "pbf_adjoint_times": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_double)]),
+    "pbf_adjoint_overflow": (ctypes.c_int, [_VP, _UP, _UP]),
+    "pbf_adjoint_scaled_gradients": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.POINTER(ctypes.c_int32)]),
     "pbf_adjoint_destroy": (None, [_VP]),
 }
 

@@ -27,6 +27,20 @@ import numpy as np
 from ._lib import check, lib
 
 
+def check_float4(t, n: int, what: str) -> None:
+    """Raises ValueError unless t is a contiguous float32 (n, 4) GPU tensor:
+    the library reads or writes n float4 rows through its data pointer."""
+    shape = tuple(getattr(t, "shape", ()))
+    if shape != (n, 4):
+        raise ValueError(f"{what} must have shape ({n}, 4), got {shape}")
+    if str(getattr(t, "dtype", "")) != "torch.float32":
+        raise ValueError(f"{what} must be float32, got {getattr(t, 'dtype', type(t))}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{what} must be a GPU tensor")
+
+
 def tape_bytes(sim) -> tuple[int, int]:
     """Device bytes of one recorded substep and of one checkpoint."""
     sim._ensure()
@@ -96,6 +110,8 @@ class Rollout:
     def forward_raw(self, x0, v0):
         """x0, v0: float32 (n, 4) contiguous GPU tensors; returns (xT, vT)."""
         import torch
+        check_float4(x0, self.n, "x0")
+        check_float4(v0, self.n, "v0")
         xT = torch.empty_like(x0)
         vT = torch.empty_like(v0)
         torch.cuda.synchronize()
@@ -106,6 +122,10 @@ class Rollout:
     def backward_raw(self, gxT=None, gvT=None):
         """Returns (dL/dx0, dL/dv0, dL/dcoef) for the last forward_raw."""
         import torch
+        if gxT is not None:
+            check_float4(gxT, self.n, "gxT")
+        if gvT is not None:
+            check_float4(gvT, self.n, "gvT")
         dev = torch.device("cuda")
         gx0 = torch.empty((self.n, 4), dtype=torch.float32, device=dev)
         gv0 = torch.empty((self.n, 4), dtype=torch.float32, device=dev)
@@ -115,6 +135,29 @@ class Rollout:
                                          None if gvT is None else gvT.data_ptr(), gx0.data_ptr(), gv0.data_ptr(),
                                          gc.data_ptr()), self._ctx)
         return gx0, gv0, gc
+
+    def overflow(self) -> np.ndarray:
+        """Per scene, bits of the gradients of the last backward that exceed
+        binary32 (returned as +-inf): 1 coefficient, 2 dL/dx0 or dL/dv0."""
+        k = ctypes.c_uint32()
+        out = np.zeros(self.scenes, np.uint32)
+        check(lib().pbf_adjoint_overflow(self._a, ctypes.byref(k), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+              self._ctx)
+        return out
+
+    def scaled_gradients(self):
+        """The last backward's gradients as (gx0, gv0, gcoef) mantissas (GPU
+        tensors) and a per-scene exponent array: true = mantissa * 2^expo."""
+        import torch
+        dev = torch.device("cuda")
+        gx0 = torch.empty((self.n, 4), dtype=torch.float32, device=dev)
+        gv0 = torch.empty((self.n, 4), dtype=torch.float32, device=dev)
+        gc = torch.empty((self.scenes, 5), dtype=torch.float32, device=dev)
+        expo = np.zeros(self.scenes, np.int32)
+        torch.cuda.synchronize()
+        check(lib().pbf_adjoint_scaled_gradients(self._a, gx0.data_ptr(), gv0.data_ptr(), gc.data_ptr(),
+                                                 expo.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), self._ctx)
+        return gx0, gv0, gc, expo
 
     def times(self) -> dict:
         f, b, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

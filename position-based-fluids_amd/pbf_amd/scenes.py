@@ -66,6 +66,27 @@ def dam_break_block(nx: int, ny: int, nz: int, h: float, seed: int, origin=(0.0,
     return out
 
 
+def reference_block(params: PbfParams) -> np.ndarray:
+    """float32[N,4] initial positions of the reference's CreateParticles
+    (src/Simulation.cpp:71-101), as the C++ drop-in places them: a cube with
+    a = ceil(N^(1/3)) particles per side at spacing d = h * setupSpacing,
+    filled y first, then x, then z (particle i at lattice point
+    (floor(i/a) mod a, i mod a, floor(i/a^2) mod a)), its x and z centred on
+    1/2 and its bottom at y = 0.3; w = 0.  Every operation is binary32 in the
+    same order, so the block is bitwise the drop-in's."""
+    n = int(params.particleCount)
+    a = int(math.ceil(math.pow(n, 1.0 / 3.0)))
+    d = np.float32(params.h) * np.float32(params.setupSpacing)
+    corner_xz = (np.float32(1.0) - np.float32(a) * d) / np.float32(2.0)
+    i = np.arange(n, dtype=np.int64)
+    lattice = ((i // a) % a, i % a, (i // (a * a)) % a)
+    corner = (corner_xz, np.float32(0.3), corner_xz)
+    out = np.zeros((n, 4), np.float32)
+    for k in range(3):
+        out[:, k] = corner[k] + lattice[k].astype(np.float32) * d
+    return out
+
+
 def morton_grid_size(box_extent, h: float) -> int:
     """Smallest power of two above every in-box Morton id (no hash folding)."""
     cells = [max(1, math.ceil(e / h)) for e in box_extent]

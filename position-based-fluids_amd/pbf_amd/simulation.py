@@ -57,9 +57,17 @@ class Simulation:
         return int(self.params.particleCount)
 
     def set_state(self, x: np.ndarray, v: np.ndarray | None = None):
+        """Positions and velocities (float (n, 4) each, n = particleCount; v
+        None for zero velocities), checked before any device work."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.shape != (self.n, 4):
+            raise ValueError(f"x must have shape ({self.n}, 4), got {x.shape}")
+        vv = None
+        if v is not None:
+            vv = np.ascontiguousarray(v, dtype=np.float32)
+            if vv.shape != x.shape:
+                raise ValueError(f"v must have shape {x.shape} like x, got {vv.shape}")
         self._ensure()
-        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 4)
-        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 4)
         check(lib().pbf_set_state(self._ctx, _ptr(x), _ptr(vv), x.shape[0]), self._ctx)
 
     def update_params(self, params: PbfParams, scene: int = 0):

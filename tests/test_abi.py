@@ -146,3 +146,33 @@ def test_friend_circle_thresholds_match_formula(h):
     r2 = r2[r2 < h2]
     got = (r2[:, None] >= t[None, :]).sum(axis=1)
     assert np.array_equal(got, circle(r2))
+
+
+def test_set_state_rejects_mismatched_shapes_before_device_work():
+    """Simulation.set_state checks shapes before it creates the context (no
+    GPU here): the library reads particleCount float4 rows from each array."""
+    scene = pbf_amd.config("dam4k")
+    sim = pbf_amd.Simulation(scene.params, scene.options)
+    n = scene.n
+    for x, v in ((np.zeros((n - 1, 4), np.float32), None),
+                 (np.zeros((n, 3), np.float32), None),
+                 (np.zeros((n, 4), np.float32), np.zeros((n - 1, 4), np.float32)),
+                 (np.zeros((n, 4), np.float32), np.zeros((n, 3), np.float32))):
+        with pytest.raises(ValueError):
+            sim.set_state(x, v)
+    assert sim._ctx is None
+
+
+def test_adjoint_buffers_are_validated():
+    """Rollout.forward_raw/backward_raw hand device pointers to the library,
+    which reads and writes n float4 rows: anything but a contiguous float32
+    (n, 4) GPU tensor is refused first."""
+    torch = pytest.importorskip("torch")
+    from pbf_amd.adjoint import check_float4
+    n = 8
+    ok_shape = torch.zeros((n, 4), dtype=torch.float32)
+    for t, what in ((torch.zeros((n - 1, 4)), "shape"), (torch.zeros((n, 3)), "shape"),
+                    (torch.zeros((n, 4), dtype=torch.float64), "float32"), (ok_shape, "GPU"),
+                    (torch.zeros((4, n)).t(), "contiguous")):
+        with pytest.raises(ValueError, match=what):
+            check_float4(t, n, "x0")

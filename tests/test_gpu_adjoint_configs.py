@@ -1,0 +1,275 @@
+"""The adjoint at the sizes of the two differentiation workloads.
+
+Inverse problem: 65,536 particles (64x32x32 block, h = 0.1), 60 substeps,
+4 iterations, checkpoint every 10; targets from a rollout with Gaussian
+initial velocities (sigma 0.5 m/s), loss mean |xT - target|^2 at v0 = 0; its
+4,096-particle copy checked against the frozen-structure checker over the
+full 60 substeps.
+
+Batched differentiation: 32 scenes (counts log-uniform in [8192, 65536]),
+100 substeps, per-scene gravity on the lower hemisphere, loss
+|centroid - target|^2 per scene, checkpoint interval planned under 64 GB.
+Its sensitivities to gravity grow about 3x per substep: the smallest scene's
+gradient is checked against the binary32 directional derivative of the
+checker (FP64 tangents, which do not overflow), and gradients whose true
+value exceeds binary32 are reported as overflowed and returned in scaled
+form.
+"""
+import numpy as np
+import pytest
+
+import pbf_amd
+from pbf_amd.adjoint import Rollout, plan_interval
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-3
+
+
+def _dev(a):
+    t = torch.zeros((a.shape[0], 4), dtype=torch.float32)
+    t[:, :3] = torch.from_numpy(np.ascontiguousarray(a[:, :3], np.float32))
+    return t.cuda().contiguous()
+
+
+# -- inverse problem, full size ----------------------------------------------
+
+
+def test_inverse_problem_full_size():
+    """65,536 particles x 60 substeps, interval 10: the adjoint's rollout is
+    the plain substep's bit for bit, every gradient is finite, and interval 1
+    gives bitwise the same gradients."""
+    scene = pbf_amd.scenes.inverse_problem(64, 32, 32, seed=0)
+    n, steps = scene.n, 60
+    assert n == 65536
+    sim = pbf_amd.Simulation(scene.params, scene.options)
+    # plain substeps, particle identities followed through every sort
+    sim.set_state(scene.x, scene.v)
+    ids = np.arange(n)
+    for _ in range(steps):
+        sim.Step()
+        ids = ids[sim.get_structure(friends=False)["perm"]]
+    st = sim.get_state()
+    xT_plain = np.empty((n, 4), np.float32)
+    vT_plain = np.empty((n, 4), np.float32)
+    xT_plain[ids] = st["x"]
+    vT_plain[ids] = st["v"]
+
+    grads = []
+    for interval in (10, 1):
+        roll = Rollout(sim, steps, interval)
+        target, vT = roll.forward_raw(_dev(scene.x), _dev(scene.v))
+        if interval == 10:
+            t = target.cpu().numpy()
+            assert np.array_equal(t[:, :3].view(np.uint32), xT_plain[:, :3].view(np.uint32))
+            assert np.array_equal(vT.cpu().numpy()[:, :3].view(np.uint32), vT_plain[:, :3].view(np.uint32))
+        x0, v0 = _dev(scene.x), _dev(np.zeros_like(scene.v))
+        xT, _ = roll.forward_raw(x0, v0)
+        g = 2.0 * (xT - target) / n
+        g[:, 3] = 0.0
+        gx0, gv0, gc = roll.backward_raw(g, None)
+        out = [a.cpu().numpy() for a in (gx0, gv0, gc)]
+        for a in out:
+            assert np.isfinite(a).all()
+        assert not roll.overflow().any()
+        assert float(np.abs(out[1]).max()) > 0.0
+        grads.append(out)
+        roll.close()
+    for a, b in zip(*grads):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    sim.close()
+
+
+class Copy4k:
+    """The inverse problem's 4,096-particle copy over the full 60 substeps,
+    interval 10, with the loss mean |xT - target|^2 + 0.1 mean |vT|^2."""
+
+    def __init__(self, oracle):
+        scene = pbf_amd.scenes.inverse_problem(16, 16, 16, seed=1)
+        self.scene = scene
+        n = scene.n
+        self.sim = pbf_amd.Simulation(scene.params, scene.options)
+        self.roll = Rollout(self.sim, 60, 10)
+        self.fr = oracle.FrozenRollout(oracle.cfg_from_params(scene.params, scene.options), scene.x, scene.v, 60)
+        xT, vT = self.roll.forward_raw(_dev(scene.x), _dev(scene.v))
+        self.xT = xT.cpu().numpy()[:, :3]
+        self.vT = vT.cpu().numpy()[:, :3]
+        tgt = self.fr.xT32 + np.random.default_rng(0).normal(0, 0.01, (n, 3)).astype(np.float32)
+        self.gxT = (2.0 * (self.xT - tgt) / n).astype(np.float32)
+        self.gvT = (0.2 * self.vT / n).astype(np.float32)
+        gx0, gv0, gc = self.roll.backward_raw(_dev(self.gxT), _dev(self.gvT))
+        self.gx0 = gx0.cpu().numpy().astype(np.float64)
+        self.gv0 = gv0.cpu().numpy().astype(np.float64)
+        self.gc = gc.cpu().numpy().astype(np.float64)
+
+    def dL(self, dxT, dvT):
+        return float(np.sum(self.gxT * dxT) + np.sum(self.gvT * dvT))
+
+    def close(self):
+        self.roll.close()
+        self.sim.close()
+
+
+@pytest.fixture(scope="module")
+def copy4k(oracle):
+    c = Copy4k(oracle)
+    yield c
+    c.close()
+
+
+def test_copy4k_rollout_is_the_binary32_rollout(copy4k):
+    assert np.array_equal(copy4k.xT, copy4k.fr.xT32)
+    assert np.array_equal(copy4k.vT, copy4k.fr.vT32)
+
+
+def test_copy4k_state_gradient_over_60_substeps(copy4k):
+    rng = np.random.default_rng(11)
+    n = copy4k.scene.n
+    dv = rng.normal(0, 1, (n, 3))
+    dx = rng.normal(0, 1e-2, (n, 3))
+    adj = float(np.sum(copy4k.gv0[:, :3] * dv) + np.sum(copy4k.gx0[:, :3] * dx))
+    _, _, dxT, dvT = copy4k.fr.jvp32(dx0=dx, dv0=dv)
+    jvp = copy4k.dL(dxT, dvT)
+    assert abs(adj - jvp) <= RTOL * abs(jvp), (adj, jvp)
+    fx, fv = copy4k.fr.tangent_fd(dx0=dx, dv0=dv)
+    fd = copy4k.dL(fx, fv)
+    assert abs(adj - fd) <= RTOL * abs(fd), (adj, fd)
+
+
+@pytest.mark.parametrize("k", range(5))
+def test_copy4k_coefficient_gradients_over_60_substeps(copy4k, k):
+    dc = np.zeros(5)
+    dc[k] = 1.0
+    _, _, dxT, dvT = copy4k.fr.jvp32(dcoef=dc)
+    jvp = copy4k.dL(dxT, dvT)
+    adj = copy4k.gc[0, k]
+    assert abs(adj - jvp) <= RTOL * abs(jvp), (k, adj, jvp)
+    fx, fv = copy4k.fr.tangent_fd(dcoef=dc)
+    fd = copy4k.dL(fx, fv)
+    assert abs(adj - fd) <= RTOL * abs(fd), (k, adj, fd)
+
+
+# -- batched differentiation, full size --------------------------------------
+
+
+class Gravity32:
+    """The 32-scene gravity batch over 100 substeps and its gradients."""
+
+    steps = 100
+
+    def __init__(self):
+        self.scenes, self.targets = pbf_amd.batch.gravity_ensemble(32, seed=0)
+        sc = self.scenes
+        self.batch = pbf_amd.BatchSimulation([s.params for s in sc], [s.options for s in sc])
+        self.interval = plan_interval(self.batch, self.steps, 64 << 30)
+        self.off = self.batch.offsets.astype(np.int64)
+        self.x = np.concatenate([s.x for s in sc])
+        self.v = np.concatenate([s.v for s in sc])
+        roll = Rollout(self.batch, self.steps, self.interval)
+        xT, _ = roll.forward_raw(_dev(self.x), _dev(self.v))
+        self.xT = xT.cpu().numpy()
+        self.g = self.loss_grad(self.xT, self.off, self.targets)
+        gx0, gv0, gc = roll.backward_raw(_dev(self.g), None)
+        self.gx0, self.gv0, self.gc = (a.cpu().numpy() for a in (gx0, gv0, gc))
+        self.ovf = roll.overflow()
+        mx, mv, mc, self.expo = roll.scaled_gradients()
+        self.mx, self.mv, self.mc = (a.cpu().numpy() for a in (mx, mv, mc))
+        roll.close()
+
+    @staticmethod
+    def loss_grad(xT, off, targets):
+        """dL/dxT of sum_s |centroid_s - target_s|^2 (float32, w = 0)."""
+        g = np.zeros((xT.shape[0], 4), np.float32)
+        for s in range(len(off) - 1):
+            a, b = off[s], off[s + 1]
+            cen = xT[a:b, :3].astype(np.float64).mean(0)
+            g[a:b, :3] = (2.0 * (cen - targets[s]) / (b - a)).astype(np.float32)
+        return g
+
+    def close(self):
+        self.batch.close()
+
+
+@pytest.fixture(scope="module")
+def gravity32():
+    g = Gravity32()
+    yield g
+    g.close()
+
+
+def test_gravity32_overflow_is_reported(gravity32):
+    """A gradient returned as inf is flagged for its scene; the scaled form is
+    finite everywhere and equals the true-unit gradients wherever those are
+    finite: true = mantissa * 2^expo[scene]."""
+    G = gravity32
+    S = len(G.scenes)
+    assert np.isfinite(G.mc).all() and np.isfinite(G.mx).all() and np.isfinite(G.mv).all()
+    for s in range(S):
+        a, b = G.off[s], G.off[s + 1]
+        coef_inf = not np.isfinite(G.gc[s]).all()
+        part_inf = not (np.isfinite(G.gx0[a:b]).all() and np.isfinite(G.gv0[a:b]).all())
+        assert bool(G.ovf[s] & 1) == coef_inf, s
+        assert bool(G.ovf[s] & 2) == part_inf, s
+        e = int(G.expo[s])
+        tiny = np.finfo(np.float32).tiny
+        # normal mantissas with normal true values: the two forms agree exactly
+        want_c = np.ldexp(G.mc[s].astype(np.float64), e)
+        fin = np.isfinite(G.gc[s]) & (np.abs(G.gc[s]) >= tiny) & (np.abs(G.mc[s]) >= tiny)
+        assert np.array_equal(G.gc[s][fin].astype(np.float64), want_c[fin]), s
+        want_v = np.ldexp(G.mv[a:b, :3].astype(np.float64), e)
+        got_v = G.gv0[a:b, :3].astype(np.float64)
+        fin = np.isfinite(got_v) & (np.abs(got_v) >= tiny) & (np.abs(G.mv[a:b, :3]) >= tiny)
+        assert np.array_equal(got_v[fin], want_v[fin]), s
+        big = np.abs(want_v) > np.finfo(np.float32).max
+        assert np.array_equal(np.isinf(got_v), big), s
+    print(f"scenes with a gradient beyond binary32: {int((G.ovf != 0).sum())} of {S}")
+
+
+def test_gravity32_batch_equals_scenes_alone(gravity32):
+    """Three scenes of the batch differentiated alone give their rows of the
+    batch's gradients bit for bit (scaled form included)."""
+    G = gravity32
+    order = np.argsort(np.diff(G.off))
+    for s in (int(order[0]), int(order[1]), int(order[-1])):
+        sc = G.scenes[s]
+        sim = pbf_amd.Simulation(sc.params, sc.options)
+        roll = Rollout(sim, G.steps, G.interval)
+        xT, _ = roll.forward_raw(_dev(sc.x), _dev(sc.v))
+        a, b = G.off[s], G.off[s + 1]
+        assert np.array_equal(xT.cpu().numpy().view(np.uint32), G.xT[a:b].view(np.uint32)), s
+        g = Gravity32.loss_grad(xT.cpu().numpy(), np.array([0, b - a]), G.targets[s:s + 1])
+        gx0, gv0, gc = (t.cpu().numpy() for t in roll.backward_raw(_dev(g), None))
+        assert np.array_equal(gx0.view(np.uint32), G.gx0[a:b].view(np.uint32)), s
+        assert np.array_equal(gv0.view(np.uint32), G.gv0[a:b].view(np.uint32)), s
+        assert np.array_equal(gc[0].view(np.uint32), G.gc[s].view(np.uint32)), s
+        mx, mv, mc, expo = roll.scaled_gradients()
+        assert int(expo[0]) == int(G.expo[s])
+        assert np.array_equal(mc.cpu().numpy()[0].view(np.uint32), G.mc[s].view(np.uint32))
+        assert np.array_equal(mv.cpu().numpy().view(np.uint32), G.mv[a:b].view(np.uint32))
+        roll.close()
+        sim.close()
+
+
+def test_gravity32_gravity_gradient_matches_binary32_jvp(gravity32, oracle):
+    """The smallest scene's 100-substep gravity gradient, from the batch,
+    equals the checker's binary32 directional derivative within 1e-3.  Its
+    magnitude (about 1e23-1e24 for this scene) is the true sensitivity of the
+    binary32 trajectory with frozen structure, not an adjoint artefact."""
+    G = gravity32
+    s = int(np.argmin(np.diff(G.off)))
+    sc = G.scenes[s]
+    a, b = G.off[s], G.off[s + 1]
+    fr = oracle.FrozenRollout(oracle.cfg_from_params(sc.params, sc.options), sc.x, sc.v, G.steps)
+    assert np.array_equal(fr.xT32, G.xT[a:b, :3])
+    gxT = G.g[a:b, :3].astype(np.float64)
+    true = np.ldexp(G.mc[s].astype(np.float64), int(G.expo[s]))
+    for k in range(2, 5):
+        dc = np.zeros(5)
+        dc[k] = 1.0
+        _, _, dxT, _ = fr.jvp32(dcoef=dc)
+        jvp = float(np.sum(gxT * dxT))
+        assert abs(true[k] - jvp) <= RTOL * abs(jvp), (k, true[k], jvp)
+        if np.isfinite(G.gc[s, k]):
+            assert abs(float(G.gc[s, k]) - jvp) <= RTOL * abs(jvp), (k, G.gc[s, k], jvp)

@@ -1,20 +1,109 @@
-"""The C++ Simulation drop-in and its headless frame loop on the GPU."""
+"""The C++ Simulation drop-in and its headless frame loop on the GPU: the
+frame loop, the render hand-off and the reference's own scenario, checked
+against the oracle."""
 import os
 import subprocess
 import time
 from pathlib import Path
 
+import numpy as np
 import pytest
+
+import pbf_amd
+import pbf_oracle
+from pbf_amd.boundaries import FORCE_PLATES, FORCE_PLATES_EDGE_BASE
 
 ROOT = Path(__file__).resolve().parent.parent
 RUN = ROOT / "position-based-fluids_amd" / "bin" / "pbf_run"
 
 pytestmark = pytest.mark.gpu
 
+# The reference's shipped scenario, assets/scenarios/dam_coarse.par:1-40
+# (values as the file holds them): 50,000 particles, h = 2, 3 solver
+# iterations, a 128,000-bucket grid (not a power of two, so distinct cells
+# fold onto one bucket).
+DAM_COARSE_PAR = """# Simulation settings
+ParticleCount           50000
+SetupSpacing            0.55
+ResetSimOnChange        1
+
+# Bounds related
+XMin                    -30.0
+XMax                    90.0
+YMin                    0.0
+YMax                    60.0
+ZMin                    -30.0
+ZMax                    30.0
+WaveGenAmp              0.3
+WaveGenFreq             0.20
+WaveGenDuty             4.0
+
+# Simulation related
+TimeStep                0.025
+SubSteps                1
+SimIterations           3
+SmoothLen               2.0
+RestDensity             1.0
+Garvity                 10.0
+Epsilon                 5.0
+VorticityFactor         5.0
+ViscosityFactor         0.01
+SurfaceTenstionK       -0.1
+SurfaceTenstionDist     0.1
+
+# Grid related
+GridBufferSize          128000
+
+# Radix related
+SegmentSize             16
+SortIterations          6
+
+# Rendering related
+ParticleRenderSize      1.0
+
+# Kernels Setup
+EnableCachedBuffers     1
+"""
+
+
+def _run(args, timeout=300):
+    r = subprocess.run([str(RUN), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def _raw(path, dtype, shape):
+    return np.fromfile(path, dtype=dtype).reshape(shape)
+
+
+def _bitwise(a, b, what):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"{what}: not bitwise equal"
+
+
+def _check_frame(dump, frame, ref, n, structure=False):
+    """The drop-in's staged render buffers (and structure) of one frame
+    against the oracle's state after the same steps."""
+    st = ref.get_state()
+    rows = (n + 2047) // 2048
+    _bitwise(_raw(dump / f"{frame}.vbo", np.float32, (n, 4)), st["x"], f"frame {frame} VBO")
+    tex = _raw(dump / f"{frame}.tex", np.float32, (rows * 2048, 4))
+    _bitwise(tex[:n], st["xs"], f"frame {frame} texture")
+    assert not tex[n:].any(), "texture padding is not zero"
+    if structure:
+        rs = ref.get_structure()
+        assert np.array_equal(_raw(dump / f"{frame}.keys", np.uint32, (n,)), rs["keys"])
+        assert np.array_equal(_raw(dump / f"{frame}.perm", np.uint32, (n,)), rs["perm"])
+        assert np.array_equal(np.fromfile(dump / f"{frame}.cells", np.uint32), rs["cells"])
+        assert np.array_equal(np.fromfile(dump / f"{frame}.friends", np.uint32), rs["friends"])
+    return st
+
 
 def test_pbf_run_frame_loop():
     assert RUN.exists(), "build first"
-    r = subprocess.run([str(RUN), str(ROOT / "tests" / "golden" / "dam4k.par"), "30", "10"],
+    r = subprocess.run([str(RUN), str(ROOT / "tests" / "golden" / "dam4k.par"), "30", "10", "--walls"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     lines = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("frame")]
@@ -30,7 +119,7 @@ def test_pbf_run_frame_loop():
 def test_pbf_run_render_staging():
     # CopyRenderBuffers into vectors and into caller-owned memory: the VBO is
     # the state's positions (w = speed), the texture has ceil(4096 / 2048) rows
-    r = subprocess.run([str(RUN), str(ROOT / "tests" / "golden" / "dam4k.par"), "20", "10", "--render"],
+    r = subprocess.run([str(RUN), str(ROOT / "tests" / "golden" / "dam4k.par"), "20", "10", "--walls", "--render"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     lines = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("render")]
@@ -44,7 +133,7 @@ def test_pbf_run_with_a_ramp():
     # from the plain run
     out = []
     for extra in ([], ["ramp"]):
-        r = subprocess.run([str(RUN), str(ROOT / "tests" / "golden" / "dam4k.par"), "20", "10", *extra],
+        r = subprocess.run([str(RUN), str(ROOT / "tests" / "golden" / "dam4k.par"), "20", "10", "--walls", *extra],
                            capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
         lines = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("frame")]
@@ -65,7 +154,7 @@ def test_pbf_run_hot_reload(tmp_path):
     text = (ROOT / "tests" / "golden" / "dam4k.par").read_text().replace("ResetSimOnChange        1",
                                                                           "ResetSimOnChange        0")
     par.write_text(text)
-    proc = subprocess.Popen([str(RUN), str(par), "600", "5", "--watch", "--fps", "100"], stdout=subprocess.PIPE,
+    proc = subprocess.Popen([str(RUN), str(par), "600", "5", "--walls", "--watch", "--fps", "100"], stdout=subprocess.PIPE,
                             stderr=subprocess.PIPE, text=True)
     lines = []
     try:
@@ -105,3 +194,120 @@ def test_pbf_run_hot_reload(tmp_path):
     # upward gravity: the top of the block rises after the reload
     assert float(after[-1][9]) > float(after[0][9]) + 0.05
     assert [f for f in fr if int(f[1]) > int(rl2[2])][0][3] == "3000"
+
+
+def test_initial_block_is_the_reference_block():
+    """reference_block (the drop-in's CreateParticles) by its closed form:
+    a = ceil(N^(1/3)) per side, particle i at lattice point
+    (floor(i/a) mod a, i mod a, floor(i/a^2) mod a) of spacing h*setupSpacing
+    from corner ((1 - a d)/2, 0.3, (1 - a d)/2) (src/Simulation.cpp:71-101)."""
+    p, _ = pbf_amd.load_parameters(DAM_COARSE_PAR)
+    x = pbf_amd.reference_block(p)
+    a = 37                                   # ceil(50000^(1/3)) = ceil(36.84)
+    d = np.float32(2.0) * np.float32(0.55)
+    c = (np.float32(1.0) - np.float32(a) * d) / np.float32(2.0)
+    assert x.shape == (50000, 4) and not x[:, 3].any()
+    for i, lat in ((0, (0, 0, 0)), (1, (0, 1, 0)), (a, (1, 0, 0)), (a * a, (0, 0, 1)), (a * a + a + 1, (1, 1, 1)),
+                   (49999, ((49999 // a) % a, 49999 % a, 49999 // (a * a)))):
+        want = [c + np.float32(lat[0]) * d, np.float32(0.3) + np.float32(lat[1]) * d, c + np.float32(lat[2]) * d]
+        assert np.array_equal(x[i, :3], np.array(want, np.float32)), i
+    # 36 full y-x layers of a^2 and a partial 37th
+    assert int(np.unique(x[:, 2]).size) == 37 and int((x[:, 2] == x[:, 2].max()).sum()) == 50000 - 36 * a * a
+
+
+def test_render_handoff_matches_oracle(tmp_path):
+    """Runner's render inputs after every frame, staged by the drop-in
+    (CopyRenderBuffers): the position VBO (w = |v|) and the 2048-wide RGBA32F
+    texture of x* (w = density) with a partial last row (3,000 particles),
+    bitwise equal to the oracle's state; the ping/pong position buffer ids
+    swap on every frame that sorts and stay on paused frames
+    (src/Simulation.cpp:209-211,533-552; src/Runner.cpp:58-60)."""
+    text = (ROOT / "tests" / "golden" / "dam4k.par").read_text()
+    text = text.replace("ParticleCount           4096", "ParticleCount           3000")
+    text = text.replace("SubSteps                2", "SubSteps                1")
+    par = tmp_path / "ragged.par"
+    par.write_text(text)
+    dump = tmp_path / "dump"
+    dump.mkdir()
+    paused = {3, 4, 7}
+    frames = 9
+    _run([par, frames, 1, "--walls", "--pause", ",".join(map(str, sorted(paused))), "--dump", dump])
+    p, _ = pbf_amd.load_parameters(text)
+    n = 3000
+    x0 = _raw(dump / "init.x", np.float32, (n, 4))
+    _bitwise(x0, pbf_amd.reference_block(p), "initial block")
+    ref = pbf_oracle.OracleSim(pbf_oracle.cfg_from_params(p, pbf_amd.default_options(walls=True)))
+    ref.set_state(x0, np.zeros_like(x0))
+    ids = (1, 2)
+    for f in range(1, frames + 1):
+        ref.step(f in paused)
+        st = _check_frame(dump, f, ref, n, structure=(f == frames))
+        if f not in paused:
+            ids = ids[::-1]
+        assert tuple(int(v) for v in np.fromfile(dump / f"{f}.ids", np.uint32)) == ids, f
+    # the texture's w is the density of the last solver iteration
+    # (pack_data.cl, update_velocities.cl:16-17)
+    _bitwise(st["xs"][:, 3], st["rho"], "texture w = density")
+    assert st["x"][:, 3].any()
+    ref.close()
+
+
+def _morton(c):
+    """Morton id of integer cell coordinates (x bit 3k, y 3k+1, z 3k+2) on
+    their 10-bit two's-complement patterns (utilities.cl:69-87), unfolded."""
+    c = np.asarray(c, np.int64) & 1023
+    m = np.zeros(c.shape[0], np.int64)
+    for b in range(10):
+        for axis in range(3):
+            m |= ((c[:, axis] >> b) & 1) << (3 * b + axis)
+    return m
+
+
+def test_reference_scenario_dam_coarse(tmp_path):
+    """The reference's own scenario through the C++ drop-in: dam_coarse.par
+    (50,000 particles, h = 2, 3 iterations, 128,000 hash buckets), the
+    CreateParticles block, the 14 force plates of compute_delta.cl:169-182
+    with their "1.0 +" edge offset over a synthetic 512 x 9,262 mask
+    (LoadForceMasks reads the PNG's red channel; random texels here), no box
+    walls.  After 20 frames the staged render buffers and the integer
+    structure equal the oracle's bit for bit, and plates move particles.
+    The 128,000-bucket table is not a power of two, so keys go through the
+    modulo (negative cell coordinates wrap to Morton ids near 2^30) and the
+    neighbour-cell culling is off; no two cells of this scene's first 20
+    frames share a bucket (folded buckets are covered by the hash-folding
+    parity tests)."""
+    par = tmp_path / "dam_coarse.par"
+    par.write_text(DAM_COARSE_PAR)
+    rng = np.random.default_rng(2013)
+    mask = rng.integers(0, 256, (9262, 512), dtype=np.uint8)
+    mask[rng.random(mask.shape) < 0.3] = 0
+    (tmp_path / "mask.u8").write_bytes(mask.tobytes())
+    dump = tmp_path / "dump"
+    dump.mkdir()
+    frames = 20
+    _run([par, frames, frames, "--plates", tmp_path / "mask.u8", "--dump", dump])
+    p, unknown = pbf_amd.load_parameters(DAM_COARSE_PAR)
+    assert unknown == 0 and p.gridBufSize == 128000 and p.simIterations == 3
+    n = 50000
+    x0 = _raw(dump / "init.x", np.float32, (n, 4))
+    _bitwise(x0, pbf_amd.reference_block(p), "initial block")
+    cfg = pbf_oracle.cfg_from_params(p, pbf_amd.default_options(walls=False))
+    pbf_oracle.set_quads(cfg, FORCE_PLATES, mask.astype(np.uint32), FORCE_PLATES_EDGE_BASE)
+    ref = pbf_oracle.OracleSim(cfg)
+    ref.set_state(x0, np.zeros_like(x0))
+    hits = np.zeros((int(p.simIterations), n), np.uint32)
+    pbf_oracle.lib().pbfo_record_walls(ref._s, pbf_oracle.ptr(hits))
+    fired = 0
+    for _ in range(frames):
+        ref.step()
+        fired |= int(np.bitwise_or.reduce(hits, axis=None))
+    pbf_oracle.lib().pbfo_record_walls(ref._s, None)
+    _check_frame(dump, frames, ref, n, structure=True)
+    ref.close()
+    plates = [k for k in range(14) if fired & (1 << (6 + k))]
+    assert len(plates) >= 2 and not (fired & 63), (plates, fired)   # plates only, no box walls
+    # the hash: Morton ids of the wrapped cell coordinates, reduced modulo
+    # 128,000 (utilities.cl:79-87); the initial cells' ids span far beyond it
+    cells = np.trunc(x0[:, :3] / np.float32(2.0)).astype(np.int64)
+    ids = _morton(cells)
+    assert p.gridBufSize & (p.gridBufSize - 1) and int(ids.max()) > 1 << 29

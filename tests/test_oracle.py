@@ -629,3 +629,119 @@ def test_pair_just_above_the_coincidence_bound_has_a_direction(oracle):
     po.lib().pbfo_delta(ctypes.byref(c), po.ptr(xs), po.ptr(xs), po.ptr(fr), po.ptr(delta))
     assert delta[0, 0] != 0.0 and delta[0, 0] == -delta[1, 0]
     assert delta[0, 1] == 0.0 and delta[0, 2] == 0.0
+
+
+# --- the whole substep (pbfo_step, src/Simulation.cpp:554-621) ---------------
+
+
+def _step_cfg(n, K, eps, visc, vort, dt=1.0 / 128.0, h=0.1, gravity=(0.0, 0.0, 0.0)):
+    c = _cfg(n=n, h=h, dt=dt, gravity=gravity, visc=visc, vort=vort)
+    c.iters = K
+    c.epsilon = eps
+    return c
+
+
+def _run_step(c, x, v):
+    """One pbfo_step from (x, v); the state in the initial particle order
+    (sorted slot i holds particle perm[i], radixsort.cl:22-37)."""
+    sim = po.OracleSim(c)
+    sim.set_state(x, v)
+    sim.step()
+    st = sim.get_state()
+    perm = sim.get_structure(friends=False)["perm"]
+    sim.close()
+    out = {}
+    for k, a in st.items():
+        out[k] = np.empty_like(a)
+        out[k][perm] = a
+    return out
+
+
+def test_step_of_an_isolated_particle_is_exact(oracle):
+    """A particle with no friends, gravity (0, -8, 0), no walls, dyadic
+    inputs: x' = x + dt v + dt^2 g and v' = v + dt g exactly, x.w = |v'|
+    (correctly rounded), density 0 in x*.w, no vorticity."""
+    dt = 1.0 / 64.0
+    c = _step_cfg(1, 4, 600.0, 0.3, 0.5, dt=dt, gravity=(0.0, -8.0, 0.0))
+    x = np.array([[1.25, 2.5, 0.75, 0.0]], np.float32)
+    v = np.array([[0.5, -0.25, 1.0, 0.0]], np.float32)
+    st = _run_step(c, x, v)
+    v1 = np.array([0.5, -0.25 - 8.0 * dt, 1.0], np.float32)
+    x1 = x[0, :3] + np.float32(dt) * v1
+    assert st["v"][0, :3].tolist() == v1.tolist() and st["v"][0, 3] == 0.0
+    assert st["x"][0, :3].tolist() == x1.tolist() == [1.25 + dt * 0.5, 2.5 + dt * -0.375, 0.75 + dt]
+    assert st["x"][0, 3] == np.sqrt(np.float32(0.25 + 0.140625 + 1.0))
+    assert st["xs"][0].tolist() == [*x1.tolist(), 0.0] and st["rho"][0] == 0.0
+    assert st["omega"][0].tolist() == [0.0, 0.0, 0.0, 0.0]
+
+
+def _pair_step_closed_form(x, v, h, dt, rho0, eps, k_s, dq, visc, vort, K, poly6, spiky):
+    """One substep of two particles within h of each other (and of nobody
+    else), no gravity, no walls, in FP64, with the sums of each stage
+    reduced to their single pair term (particle 1's terms are particle 0's
+    with r -> -r).  Stage order of src/Simulation.cpp:554-621: predict; K x
+    (density and lambda, delta p with s_corr); velocity update; XSPH (Jacobi)
+    with omega; vorticity confinement from this substep's omega.  Also
+    returns the vorticity and XSPH increments (the size of each stage)."""
+    x = x[:, :3].astype(np.float64)
+    v = v[:, :3].astype(np.float64)
+    xs = x + dt * v
+    for _ in range(K):
+        r = xs[0] - xs[1]
+        rl = np.linalg.norm(r)
+        w = h * h - rl * rl
+        rho = poly6 * w ** 3                                   # self excluded
+        g = spiky * (h - rl) ** 2                              # |grad W|
+        lam = -(rho / rho0 - 1.0) / (2.0 * g * g / (rho0 * rho0) + eps)
+        s = k_s * (w ** 3 / (h * h - (dq * h) ** 2) ** 3) ** 4
+        dp = -spiky * (2.0 * lam + s) * (h - rl) ** 2 * (r / rl) / rho0
+        xs = np.array([xs[0] + dp, xs[1] - dp])
+    vel = (xs - x) / dt
+    speed = np.linalg.norm(vel, axis=1)
+    r = xs[0] - xs[1]
+    rl = np.linalg.norm(r)
+    psi = r / rl * (h - rl) ** 2                              # particle 0's; particle 1's is -psi
+    u = vel[1] - vel[0]                                        # v_j - v_i for particle 0
+    om = -spiky * np.cross(u, psi)                             # the same for both
+    dvx = visc * poly6 * (h * h - rl * rl) ** 3 * u
+    vx = np.array([vel[0] + dvx, vel[1] - dvx])
+    n0 = -r / rl                                               # eta = -SPIKY |w| psi, normalised
+    dvv = np.array([dt * vort * np.cross(n0, om), dt * vort * np.cross(-n0, om)])
+    out = {
+        "x": np.column_stack([xs, speed]),
+        "xs": np.column_stack([xs, [rho, rho]]),
+        "v": np.column_stack([vx + dvv, np.zeros(2)]),         # v.w is never written
+        "omega": np.column_stack([[om, om], [np.linalg.norm(om)] * 2]),
+    }
+    return out, float(np.abs(dvv).max()), float(np.abs(dvx).max())
+
+
+@pytest.mark.parametrize("eps", [600.0, 1e30])
+def test_step_of_a_pair_matches_its_closed_form(oracle, eps):
+    """The whole substep of an isolated pair with K = 2 against its FP64
+    closed form, all four components of x (w = |v| after the velocity
+    update), x* (w = the last iteration's density), v and omega (w = |omega|).
+    Each stage moves the result by far more than the tolerance, so a
+    reordering (vorticity before XSPH, XSPH on the velocities before the
+    update), a stale omega, or lambda in place of the density in x*.w fails
+    it.  With eps = 1e30 every |lambda| is below 1e-8: XSPH and vorticity
+    then only act because their neighbour filter reads the density from
+    x*.w (apply_viscosity.cl, apply_vorticity.cl), not lambda."""
+    h, dt, K = 0.1, 1.0 / 128.0, 2
+    visc, vort = 0.003, 0.003
+    c = _step_cfg(2, K, eps, visc, vort, dt=dt, h=h)
+    x = np.array([[0.5, 0.5, 0.5, 0.0], [0.55, 0.5, 0.51, 0.0]], np.float32)
+    v = np.array([[0.0, 0.4, 0.1, 0.0], [0.1, -0.4, 0.0, 0.0]], np.float32)
+    st = _run_step(c, x, v)
+    poly6, spiky = oracle.kernel_factors(h)
+    want, d_vort, d_xsph = _pair_step_closed_form(x, v, h, dt, float(c.rest_density), eps, float(c.st_k),
+                                                  float(c.st_dist), visc, vort, K, poly6, spiky)
+    for k in ("x", "xs", "v", "omega"):
+        got = st[k].astype(np.float64)
+        tol = 2e-4 * float(np.abs(want[k]).max())
+        np.testing.assert_allclose(got, want[k], rtol=2e-4, atol=tol, err_msg=k)
+        if k == "v":
+            assert d_vort > 50 * tol and d_xsph > 50 * tol, (d_vort, d_xsph, tol)
+    assert np.array_equal(st["rho"], st["xs"][:, 3])
+    if eps == 1e30:
+        assert float(np.abs(st["lambda"]).max()) < 1e-8

@@ -130,6 +130,24 @@ int pbf_step(pbf_ctx *ctx, uint32_t pause);
  * (src/Runner.cpp:124-132). */
 int pbf_run(pbf_ctx *ctx, uint32_t steps, uint32_t pause);
 
+/* pbf_run for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Runs `steps` substeps of one scene from
+ * positions x4 and velocities v4 (v4 may be NULL for zero) and writes the
+ * final state to x4_out and v4_out in the cell-sorted order pbf_get_state
+ * gives; all four are DEVICE arrays of n float4.  No context is created and
+ * nothing is allocated: every other buffer of the substep lives in one
+ * static device array of the library (about 36 MB, held on each device the
+ * library runs on until the process ends), which limits a call to
+ * n <= 16384 particles and gridBufSize <= 65536 (PBF_E_INVALID beyond);
+ * there are no collision quads.  The work is enqueued on `stream` (a
+ * hipStream_t; NULL for the null stream) and the call returns without
+ * waiting.  Calls share the array of their device: two calls whose work
+ * overlaps on one device race.  Results equal pbf_create + pbf_set_state +
+ * pbf_run + pbf_get_state bit for bit.  Errors: pbf_last_error(NULL). */
+int pbf_run_noalloc(const pbf_params *params, const pbf_options *opt, const float *x4, const float *v4, uint32_t n,
+                    uint32_t steps, float *x4_out, float *v4_out, void *stream);
+
 /* Replays each substep as an instantiated HIP graph (captured on first use
  * for every variant of the buffer rotation).  Ignored while profiling. */
 int pbf_set_graph(pbf_ctx *ctx, int enable);

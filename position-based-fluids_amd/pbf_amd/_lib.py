@@ -191,6 +191,8 @@ _SIGNATURES = {
                                                ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]),
     "pbf_step": (ctypes.c_int, [_VP, ctypes.c_uint32]),
     "pbf_run": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32]),
+    "pbf_run_noalloc": (ctypes.c_int, [ctypes.POINTER(PbfParams), ctypes.POINTER(PbfOptions), _VP, _VP,
+                                       ctypes.c_uint32, ctypes.c_uint32, _VP, _VP, _VP]),
     "pbf_sync": (ctypes.c_int, [_VP]),
     "pbf_get_state": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "pbf_get_structure": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),

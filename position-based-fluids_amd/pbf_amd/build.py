@@ -40,7 +40,8 @@ HIP_FLAGS = [
     f"-I{REPO_ROOT / 'include'}",
 ]
 
-SOURCES = ["pbf_sort.hip", "pbf_forward.hip", "pbf_capi.hip", "pbf_adjoint.hip", "pbf_probe.hip", "pbf_params.cpp"]
+SOURCES = ["pbf_sort.hip", "pbf_forward.hip", "pbf_capi.hip", "pbf_adjoint.hip", "pbf_probe.hip", "pbf_noalloc.hip",
+           "pbf_params.cpp"]
 
 
 def hipcc() -> str:

@@ -179,10 +179,15 @@ class Simulation:
         arr = (ctypes.c_uint32 * len(FAMILIES))(*[int(sizes[f]) for f in FAMILIES])
         check(lib().pbf_set_workgroup_sizes(self._ctx, arr), self._ctx)
 
-    def set_friends_mode(self, staged: bool):
-        """Friends build variant: LDS-staged neighbourhood or global reads."""
+    FRIENDS_MODES = {"direct": 0, "staged": 1}
+
+    def set_friends_mode(self, mode):
+        """Friends build variant (pbf_set_friends_mode): "direct" (the
+        default: candidates read from global memory) or "staged" (the tile's
+        neighbourhood in LDS); a bool selects staged (True) or direct."""
         self._ensure()
-        check(lib().pbf_set_friends_mode(self._ctx, 1 if staged else 0), self._ctx)
+        m = self.FRIENDS_MODES[mode] if isinstance(mode, str) else (1 if mode else 0)
+        check(lib().pbf_set_friends_mode(self._ctx, m), self._ctx)
 
     def set_tiles_per_workgroup(self, tiles: dict):
         """Tiles each workgroup of a family handles in turn: 1 (one per tile),

@@ -555,6 +555,7 @@ def test_strided_tiles_under_graph_replay():
     for tiles, graph in ((1, False), (4, True)):
         sim = pbf_amd.Simulation(scene.params, scene.options)
         sim.set_state(scene.x, scene.v)
+        sim.set_friends_mode("direct")
         sim.set_workgroup_sizes({f: 64 for f in fams})
         sim.set_tiles_per_workgroup({f: tiles for f in fams})
         sim.set_graph(graph)
@@ -575,6 +576,7 @@ def test_persistent_grid_strides_at_1m():
     for t in (1, 0):
         sim = pbf_amd.Simulation(scene.params, scene.options)
         sim.set_state(scene.x, scene.v)
+        sim.set_friends_mode("direct")
         sim.set_workgroup_sizes({f: 64 for f in fams})
         sim.set_tiles_per_workgroup({f: t for f in fams})
         sim.run(2)
@@ -615,7 +617,7 @@ def test_sparse_tiles_fall_back_to_global_scan():
     p = pbf_amd.physical_params(n, 0.1, (0, 0, 0), (6.0, 6.0, 6.0))
     scene = pbf_amd.Scene(p, pbf_amd.default_options(), x, np.zeros_like(x))
     sim, ref = make_pair(scene)
-    sim.set_friends_mode(True)
+    sim.set_friends_mode("staged")
     advance(sim, ref, 2)
     assert sim.stats()["unstaged_tiles"] > 0
     check_structure(sim, ref)
@@ -623,12 +625,15 @@ def test_sparse_tiles_fall_back_to_global_scan():
     sim.close()
 
 
+@pytest.mark.parametrize("mode", ["direct", "staged"])
 @pytest.mark.parametrize("size", [64, 128, 256, 512])
-def test_staged_friends_build_matches(size):
-    """The LDS-staged friends build gives the oracle's lists at every tile size."""
+def test_per_particle_friends_builds_match(mode, size):
+    """Both friends builds (global reads, LDS-staged) give the oracle's
+    lists at every tile size."""
     scene = pbf_amd.config("dam16k", seed=14)
     sim, ref = make_pair(scene)
-    sim.set_friends_mode(True)
+    sim.set_friends_mode(mode)
+    assert sim.stats()["friends_staged"] == sim.FRIENDS_MODES[mode]
     sim.set_workgroup_sizes({f: size for f in pbf_amd._lib.FAMILIES})
     advance(sim, ref, 3)
     check_structure(sim, ref)
@@ -644,3 +649,17 @@ def test_fast_sqrt_and_reciprocal_are_ieee():
     fa, fb = ctypes.c_uint32(), ctypes.c_uint32()
     assert pbf_amd.lib().pbf_check_fast_math(ctypes.byref(a), ctypes.byref(b), ctypes.byref(fa), ctypes.byref(fb)) == 0
     assert (a.value, b.value) == (0, 0), (a.value, hex(fa.value), b.value, hex(fb.value))
+
+
+@pytest.mark.parametrize("mode", ["direct", "staged"])
+def test_friends_modes_agree_at_1m(mode):
+    """Both friends build variants give the oracle's lists on the
+    1,048,576-particle dam break after a step (the lists, not just the state:
+    the sweeps only read the processed prefix)."""
+    scene = pbf_amd.config("dam1m", seed=1)
+    sim, ref = make_pair(scene)
+    sim.set_friends_mode(mode)
+    advance(sim, ref, 1)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()

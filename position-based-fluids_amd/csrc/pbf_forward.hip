@@ -450,7 +450,9 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
     __shared__ uint32_t s_stage[4 * kCircles * BS];
     for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
-        const SceneConsts& c = scenes[be.x];
+        // a copy in registers: the list stores could alias the constants'
+        // memory as far as the compiler knows, which would reload them
+        const SceneConsts c = scenes[be.x];
         const uint32_t i = be.y + threadIdx.x;
         if (i < be.z) {
             uint4* list = reinterpret_cast<uint4*>(nb_list);
@@ -499,7 +501,7 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
     __shared__ uint32_t s_stage[4 * kCircles * BS];
 
     const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
-    const SceneConsts& c = scenes[be.x];
+    const SceneConsts c = scenes[be.x];   // in registers (see k_friends_direct)
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const uint32_t i = be.y + tid;
     const bool valid = i < be.z;

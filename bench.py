@@ -8,8 +8,9 @@ then EXACTLY K timed substeps bracketed by stream synchronisation; the
 inputs are resident in HBM before timing starts.
 
 Prints one JSON line with the metric, the roofline of the dominant kernel
-(HIP events on the library's stream over the timed region) and a CPU
-baseline (the oracle restatement on this host, bounded sample).
+(HIP events that the kernels themselves stamp at their start and end, over a
+profiled replay of the timed substeps) and a CPU baseline (the oracle
+restatement on this host, bounded sample).
 """
 from __future__ import annotations
 
@@ -38,15 +39,26 @@ STAGE_BYTES = {
 
 
 # HBM bytes per launch measured with rocprofv3 PMC counters on the
-# 1,048,576-particle dam break (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a
-# plain 256 MiB copy gives the two counters), averaged over the K launches of
-# a substep for computeDelta (the last one also writes v and x);
-# profiles/pmc_traffic_dam1m.md holds the counts and commands.  They belong
-# to the kernel sources whose hash is stamped here (KERNEL_SOURCES): with any
-# other source the line reports traffic as null.
+# 1,048,576-particle dam break, over the launches of this script's profiled
+# pass (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a plain 256 MiB copy
+# gives the two counters), averaged over a family's launches (computeDelta:
+# three per substep plus the last, which also writes v and x);
+# profiles/pmc_traffic_dam1m.md holds the counts and commands (the
+# --pmc-summary mode of this script prints this table).  They belong to the
+# kernel sources whose hash is stamped here (KERNEL_SOURCES): with any other
+# source the line reports traffic as null.
 PMC_TRAFFIC = {
-    "source_sha16": "",
-    "bytes_per_launch": {},
+    "source_sha16": "0a5fb80c1dd5cda1",
+    "bytes_per_launch": {
+        "applyViscosity": 217137251,
+        "applyVorticity": 212663212,
+        "buildFriendsList": 921071385,
+        "computeDelta": 199333238,
+        "computeScaling": 198288699,
+        "onesweep": 17344987,
+        "predictPositions": 43911365,
+        "sortParticles": 96392355,
+    },
 }
 KERNEL_SOURCES = ("pbf_forward.hip", "pbf_device.h", "pbf_sort.hip", "pbf_internal.h")
 
@@ -408,6 +420,193 @@ def batched_adjoint(seed: int, steps: int = 100, budget_gb: float = 64.0, reps: 
     }
 
 
+# ---------------------------------------------------------------------------
+# rocprofv3 summaries.  The bench brackets its profiled pass (the launches its
+# roofline and stage times come from) with two tiny k_copy_probe dispatches,
+# launched after a device synchronisation; nothing else in a bench run
+# launches that kernel.  A kernel trace or counter run of the bench is cut to
+# the dispatches between the two groups, which are the line's launches.
+# ---------------------------------------------------------------------------
+WINDOW_MARK_KERNEL = "k_copy_probe"
+TARE_TRACKER, TARE_KERNEL = "eventTare", "k_tare"   # the empty kernel of a profiled substep
+
+KERNEL_FAMILIES = (
+    ("k_friends_direct<", "buildFriendsList"), ("k_friends<", "buildFriendsList"),
+    ("k_lambda<", "computeScaling"), ("k_delta<", "computeDelta"), ("k_viscosity<", "applyViscosity"),
+    ("k_vorticity<", "applyVorticity"), ("k_onesweep", "onesweep"), ("k_predict_hash", "predictPositions"),
+    ("k_predict_only", "predictPositions"), ("k_gather_cells", "sortParticles"),
+)
+
+
+def _trace_mark() -> None:
+    from pbf_amd._lib import copy_probe
+    copy_probe(1 << 16, 1)
+
+
+def _kernel_short(name: str) -> str:
+    """'void pbf::(anonymous namespace)::k_delta<256, false, false, false>(...)' -> 'k_delta<256, false, false, false>'."""
+    base = name[5:] if name.startswith("void ") else name
+    for ns in ("pbf::(anonymous namespace)::", "(anonymous namespace)::", "pbf::"):
+        base = base.replace(ns, "")
+    return base.split("(")[0].strip()
+
+
+def _family_of_kernel(short: str) -> str | None:
+    for prefix, fam in KERNEL_FAMILIES:
+        if short.startswith(prefix):
+            return fam
+    return None
+
+
+def _read_dispatches(run_dir: str, counter: str | None = None) -> list[dict]:
+    """Dispatches of a rocprofv3 CSV run (--output-format csv): kernel,
+    start, end (ns) and, for a counter run, the counter's value."""
+    import csv
+    import glob
+    pat = "*counter_collection.csv" if counter else "*kernel_trace.csv"
+    files = sorted(glob.glob(os.path.join(run_dir, "**", pat), recursive=True))
+    if not files:
+        raise SystemExit(f"no {pat} under {run_dir}")
+    rows = []
+    for f in files:
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if counter and r.get("Counter_Name") != counter:
+                    continue
+                rows.append({"kernel": _kernel_short(r["Kernel_Name"]), "start": int(r["Start_Timestamp"]),
+                             "end": int(r["End_Timestamp"]),
+                             "value": float(r["Counter_Value"]) if counter else None})
+    rows.sort(key=lambda d: d["start"])
+    return rows
+
+
+def _window(rows: list[dict]) -> tuple[int, int]:
+    """(end of the first marker group, start of the second)."""
+    marks = [d for d in rows if d["kernel"].startswith(WINDOW_MARK_KERNEL)]
+    if len(marks) < 2:
+        raise SystemExit("the run holds no window markers (was it bench.py's forward line?)")
+    gaps = [(marks[q + 1]["start"] - marks[q]["end"], q) for q in range(len(marks) - 1)]
+    _, q = max(gaps)
+    return marks[q]["end"], marks[q + 1]["start"]
+
+
+def _in_window(rows: list[dict], keep: str | None = None) -> list[dict]:
+    """The pass's dispatches of the stage kernels (or of kernel `keep`)."""
+    lo, hi = _window(rows)
+    sel = (lambda k: k.startswith(keep)) if keep else _family_of_kernel
+    return [d for d in rows if lo <= d["start"] and d["end"] <= hi and sel(d["kernel"])]
+
+
+def trace_summary(run_dir: str, line_path: str | None) -> str:
+    import csv
+    import glob
+    rows = _read_dispatches(run_dir)
+    out = []
+    stats = sorted(glob.glob(os.path.join(run_dir, "**", "*kernel_stats.csv"), recursive=True))
+    if stats:   # rocprofv3 --stats, as written
+        out.append("Whole run, every dispatch (`--stats` summary, microseconds):\n")
+        out.append("| Kernel | Calls | Total (us) | Average (us) | % |\n|---|---:|---:|---:|---:|")
+        with open(stats[0], newline="") as fh:
+            for r in list(csv.DictReader(fh))[:24]:
+                out.append(f"| `{_kernel_short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e3:.1f} | "
+                           f"{float(r['AverageNs']) / 1e3:.3f} | {float(r['Percentage']):.2f} |")
+    win = _in_window(rows)
+    fam: dict[str, list[float]] = {}
+    per_k: dict[str, list[float]] = {}
+    for d in win:
+        us = (d["end"] - d["start"]) / 1e3
+        fam.setdefault(_family_of_kernel(d["kernel"]), []).append(us)
+        per_k.setdefault(d["kernel"], []).append(us)
+    out.append("\nThe bench line's profiled pass (dispatches between the window markers):\n")
+    out.append("| Kernel | Calls | Average (us) |\n|---|---:|---:|")
+    for k, v in sorted(per_k.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"| `{k}` | {len(v)} | {sum(v) / len(v):.3f} |")
+    line = None
+    if line_path:
+        with open(line_path) as fh:
+            for ln in fh:
+                ln = ln.strip()
+                if ln.startswith("{") and '"metric"' in ln:
+                    line = json.loads(ln)
+    tare_tr = [(d["end"] - d["start"]) / 1e3 for d in _in_window(rows, keep=TARE_KERNEL)]
+    if tare_tr:
+        ev = line["roofline"].get("empty_kernel_ms") if line else None
+        out.append(f"\nThe empty `{TARE_KERNEL}` launches of the pass: {len(tare_tr)}, {np.mean(tare_tr):.3f} us each "
+                   "in the trace" + (f", {ev * 1e3:.3f} us by the line's events." if ev else "."))
+    out.append("\n| Family | Launches | Trace average (us) | Line average (us) | Line - trace (us) | Trace / line |\n"
+               "|---|---:|---:|---:|---:|---:|")
+    for f, v in sorted(fam.items(), key=lambda kv: -sum(kv[1])):
+        avg = sum(v) / len(v)
+        la = None
+        if line:
+            bk = line["roofline"]["by_kernel"].get(f)
+            la = bk["avg_launch_ms"] * 1e3 if bk else (line["roofline"]["avg_launch_ms"] * 1e3
+                                                         if line["roofline"]["kernel"] == f else None)
+        out.append(f"| {f} | {len(v)} | {avg:.3f} | {la:.3f} | {la - avg:.3f} | {avg / la:.4f} |" if la else
+                   f"| {f} | {len(v)} | {avg:.3f} | - | - | - |")
+    if line:
+        r = line["roofline"]
+        v = fam.get(r["kernel"])
+        if v:
+            avg_s = sum(v) / len(v) * 1e-6
+            frac = r["algorithmic_bytes_per_launch"] / avg_s / 1e9 / r["peak"]
+            out.append(f"\nRoofline kernel {r['kernel']}: {r['algorithmic_bytes_per_launch'] / 1e6:.3f} MB "
+                       f"algorithmic per launch / {avg_s * 1e6:.3f} us (trace) = {frac:.5f} of {r['peak']:.0f} GB/s; "
+                       f"the line reports frac {r['frac']:.5f} (ratio {frac / r['frac']:.4f}).")
+    return "\n".join(out)
+
+
+def pmc_summary(fetch_dir: str, write_dir: str, copy_fetch_dir: str, copy_write_dir: str, copy_mib: int,
+                line_path: str | None) -> str:
+    """Per-launch HBM bytes of each family over the bench line's profiled
+    pass, with FETCH_SIZE and WRITE_SIZE scaled by what they read for a plain
+    copy of copy_mib MiB (copy_mib MiB read and as many written)."""
+    kib = copy_mib * 1024.0
+    cf = [d["value"] for d in _read_dispatches(copy_fetch_dir, "FETCH_SIZE") if d["kernel"].startswith("k_copy_probe")]
+    cw = [d["value"] for d in _read_dispatches(copy_write_dir, "WRITE_SIZE") if d["kernel"].startswith("k_copy_probe")]
+    # the probe's warm-up dispatch runs first; the timed ones follow
+    f_cal, w_cal = kib / float(np.median(cf[1:] or cf)), kib / float(np.median(cw[1:] or cw))
+    out = [f"Calibration: a plain copy of {copy_mib} MiB ({kib:.0f} KiB read and written) reads FETCH_SIZE = "
+           f"{float(np.median(cf[1:] or cf)):.0f} KiB and WRITE_SIZE = {float(np.median(cw[1:] or cw)):.0f} KiB per "
+           f"dispatch: HBM bytes = (FETCH_SIZE x {f_cal:.3f} + WRITE_SIZE x {w_cal:.3f}) x 1024.\n"]
+    fw = {"FETCH_SIZE": _in_window(_read_dispatches(fetch_dir, "FETCH_SIZE")),
+          "WRITE_SIZE": _in_window(_read_dispatches(write_dir, "WRITE_SIZE"))}
+    per_k: dict[str, dict[str, list[float]]] = {}
+    fam: dict[str, dict[str, list[float]]] = {}
+    for cname, rows in fw.items():
+        for d in rows:
+            per_k.setdefault(d["kernel"], {}).setdefault(cname, []).append(d["value"])
+            fam.setdefault(_family_of_kernel(d["kernel"]), {}).setdefault(cname, []).append(d["value"])
+    # each pass autotunes on its own, so a family's kernel may differ between
+    # the two passes (another workgroup size): kernels are listed per pass
+    # and the figures bench.py uses are per family
+    out.append("| Kernel | FETCH_SIZE dispatches | FETCH_SIZE (KiB) | WRITE_SIZE dispatches | WRITE_SIZE (KiB) |\n"
+               "|---|---:|---:|---:|---:|")
+    for k, v in sorted(per_k.items(), key=lambda kv: kv[0]):
+        f, w = v.get("FETCH_SIZE"), v.get("WRITE_SIZE")
+        out.append(f"| `{k}` | {len(f) if f else 0} | {np.mean(f):.1f} | " if f else f"| `{k}` | 0 | - | ")
+        out[-1] += f"{len(w)} | {np.mean(w):.1f} |" if w else "0 | - |"
+
+    def mb(v: dict) -> float:
+        return (np.mean(v["FETCH_SIZE"]) * f_cal + np.mean(v["WRITE_SIZE"]) * w_cal) * 1024 / 1e6
+
+    out.append("\n| Family | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | HBM MB per launch |\n|---|---:|---:|---:|")
+    for f, v in sorted(fam.items(), key=lambda kv: -mb(kv[1]) if len(kv[1]) == 2 else 0):
+        if len(v) == 2:
+            out.append(f"| {f} | {np.mean(v['FETCH_SIZE']):.1f} | {np.mean(v['WRITE_SIZE']):.1f} | {mb(v):.2f} |")
+    sha = kernel_source_sha16()
+    if line_path:
+        with open(line_path) as fh:
+            for ln in fh:
+                if ln.strip().startswith("{") and '"metric"' in ln:
+                    sha = json.loads(ln)["roofline"]["kernel_source_sha16"]
+    table = {f: round(mb(v) * 1e6) for f, v in sorted(fam.items()) if "FETCH_SIZE" in v and "WRITE_SIZE" in v}
+    out.append("\nPer launch by family (averaged over the pass's launches), the table bench.py carries:\n")
+    out.append("```\nPMC_TRAFFIC = " + json.dumps({"source_sha16": sha, "bytes_per_launch": table}, indent=4)
+               + "\n```")
+    return "\n".join(out)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,7 +622,19 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint and batched-differentiation lines")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain HBM copy of MIB MiB (calibration under rocprofv3) and print GB/s")
+    ap.add_argument("--trace-summary", metavar="DIR",
+                    help="only summarise a rocprofv3 --kernel-trace CSV run of this script (no GPU needed)")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="only summarise FETCH_SIZE / WRITE_SIZE runs of this script and of --copy-probe")
+    ap.add_argument("--copy-mib", type=int, default=256, help="the --copy-probe size of the --pmc-summary runs")
+    ap.add_argument("--line", metavar="FILE", help="output of the profiled run (its JSON line), for the summaries")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary, args.line))
+        return 0
+    if args.pmc_summary:
+        print(pmc_summary(*args.pmc_summary, args.copy_mib, args.line))
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -452,14 +663,18 @@ def main() -> int:
     sim.run(args.steps)
     sim.sync()
     el = time.perf_counter() - t0
-    # the same substeps again from the same state (bitwise the same work), with
-    # HIP events around every launch on the context's stream: the per-stage
-    # breakdown and the roofline kernel's launch durations
+    # the same substeps again from the same state (bitwise the same work), each
+    # launch stamping a pair of HIP events at its kernel's start and end: the
+    # per-stage breakdown and the roofline kernel's launch durations.  The
+    # window markers let a rocprofv3 run of this script find these dispatches.
     sim.set_state(start["x"], start["v"])
     sim.set_profiling(True)
     sim.reset_timings()
+    sim.sync()
+    _trace_mark()
     sim.run(args.steps)
     sim.sync()
+    _trace_mark()
     timings = sim.timings()
     sim.set_profiling(False)
     flags, dropped = sim.error_flags()
@@ -473,6 +688,10 @@ def main() -> int:
         passes += 1
     passes = max(passes, 1)
 
+    # an empty kernel stamped the same way, once per profiled substep: the
+    # fixed part of every launch's event duration (reported, not subtracted)
+    tare = timings.pop(TARE_TRACKER, None)
+    tare_ms = tare["sum_ms"] / tare["calls"] if tare and tare["calls"] else None
     fam_ms: dict[str, float] = {}
     for name, t in timings.items():
         fam_ms[family(name)] = fam_ms.get(family(name), 0.0) + t["sum_ms"]
@@ -546,6 +765,7 @@ def main() -> int:
             "kernel_source_sha16": kernel_source_sha16(),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
+            "empty_kernel_ms": tare_ms,
             "top_kernel_overall": top,
             "by_kernel": by_kernel,
         },

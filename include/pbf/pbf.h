@@ -185,7 +185,11 @@ int pbf_device_buffers(pbf_ctx *ctx, pbf_dev_bufs *out);
 
 /* Per-stage timing, the OCLPerfMon trackers (src/OCLPerfMon.cpp:3-50):
  * same names, "_<iteration>" suffix, EMA weight 0.5.  Enabled with
- * pbf_set_profiling(ctx, 1); pbf_timings synchronises. */
+ * pbf_set_profiling(ctx, 1); pbf_timings synchronises.  Each duration is
+ * that of a pair of HIP events the kernel stamps at its start and end
+ * (hipExtLaunchKernelGGL).  The extra tracker "eventTare" times an empty
+ * kernel launched the same way once per profiled substep: the fixed part of
+ * every launch's duration. */
 typedef struct pbf_timing {
     char name[48];
     double last_ms;    /* last measured duration                    */
@@ -273,6 +277,7 @@ int pbf_friend_circle_thresholds(float h, float *t4);
  * reference: it calibrates the roofline (sustained copy bandwidth) and the
  * rocprofv3 FETCH_SIZE / WRITE_SIZE counters (kernel k_copy_probe). */
 int pbf_copy_probe(uint64_t bytes, uint32_t reps, double *gbps);
+
 
 /* Compares, on the device, the square root and reciprocal the sweeps use
  * (hardware estimate plus one FMA correction) with IEEE sqrtf and 1.0f/x:

@@ -351,6 +351,7 @@ int pbf::enqueue_step(pbf_ctx* ctx, uint32_t pause) {
     }
     HIP_TRY(ctx, pbf::launch_viscosity(ctx));
     HIP_TRY(ctx, pbf::launch_vorticity(ctx));
+    if (ctx->profiling) HIP_TRY(ctx, pbf::launch_tare(ctx));
     return PBF_OK;
 }
 
@@ -431,12 +432,13 @@ void track_begin(pbf_ctx* ctx, const char* name, int index) {
     }
     EventPair& e = ctx->events[ctx->events_used++];
     e.tracker = tid;
-    (void)hipEventRecord(e.start, ctx->stream);
+    ctx->launch_start = e.start;   // the next launch stamps them
+    ctx->launch_stop = e.stop;
 }
 
 void track_end(pbf_ctx* ctx) {
-    if (!ctx->profiling || ctx->events_used == 0) return;
-    (void)hipEventRecord(ctx->events[ctx->events_used - 1].stop, ctx->stream);
+    ctx->launch_start = nullptr;
+    ctx->launch_stop = nullptr;
 }
 
 // Reads every recorded event pair (synchronising) into the trackers: the

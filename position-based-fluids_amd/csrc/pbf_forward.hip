@@ -875,7 +875,7 @@ hipError_t query_residency(pbf_ctx* ctx) {
 
 hipError_t launch_predict_hash(pbf_ctx* ctx, bool reset_old) {
     track_begin(ctx, "predictPositions");
-    hipLaunchKernelGGL(k_predict_hash, dim3(ctx->bt_predict.count), dim3(kPredictThreads), 0, ctx->stream, ctx->sc_d,
+    PBF_KERNEL(k_predict_hash, dim3(ctx->bt_predict.count), dim3(kPredictThreads), 0, ctx->stream, ctx->sc_d,
                        ctx->bt_predict.d, ctx->x, ctx->v, ctx->keys_in, ctx->keys_sorted,
                        reinterpret_cast<uint2*>(ctx->cells), reset_old ? 1u : 0u, ctx->total_cells, ctx->sort.hist,
                        ctx->sort.passes, ctx->pause_now, ctx->err);
@@ -886,7 +886,7 @@ hipError_t launch_predict_hash(pbf_ctx* ctx, bool reset_old) {
 hipError_t launch_predict_only(pbf_ctx* ctx, uint32_t pause) {
     track_begin(ctx, "predictPositions");
     const BlockTable& t = ctx->bt[2];
-    hipLaunchKernelGGL(k_predict_only, dim3(t.count), dim3(256), 0, ctx->stream, ctx->sc_d, t.d, ctx->x, ctx->v,
+    PBF_KERNEL(k_predict_only, dim3(t.count), dim3(256), 0, ctx->stream, ctx->sc_d, t.d, ctx->x, ctx->v,
                        ctx->xs[0], pause);
     track_end(ctx);
     ctx->xs_cur = 0;
@@ -896,7 +896,7 @@ hipError_t launch_predict_only(pbf_ctx* ctx, uint32_t pause) {
 hipError_t launch_gather(pbf_ctx* ctx) {
     track_begin(ctx, "sortParticles");
     const BlockTable& t = ctx->bt[2];
-    hipLaunchKernelGGL(k_gather_cells, dim3(t.count), dim3(256), 0, ctx->stream, ctx->sc_d, t.d, ctx->perm, ctx->x,
+    PBF_KERNEL(k_gather_cells, dim3(t.count), dim3(256), 0, ctx->stream, ctx->sc_d, t.d, ctx->perm, ctx->x,
                        ctx->v, ctx->x_alt, ctx->xs[0], ctx->keys_sorted, ctx->cells, ctx->total_cells, ctx->sort.ws,
                        ctx->sort.ws_words, ctx->pause_now, ctx->n, ctx->err);
     track_end(ctx);
@@ -917,9 +917,9 @@ hipError_t launch_gather(pbf_ctx* ctx) {
         const uint32_t grid_ = launch_grid(ctx, FAM, t.count);                                                 \
         PBF_DISPATCH_BS(ctx->bs_log2[FAM], {                                                                   \
             if (grid_ < t.count)                                                                               \
-                hipLaunchKernelGGL((KERNEL, true>), dim3(grid_), dim3(BS), 0, ctx->stream, __VA_ARGS__, t.count);  \
+                PBF_KERNEL((KERNEL, true>), dim3(grid_), dim3(BS), 0, ctx->stream, __VA_ARGS__, t.count);  \
             else                                                                                               \
-                hipLaunchKernelGGL((KERNEL, false>), dim3(grid_), dim3(BS), 0, ctx->stream, __VA_ARGS__, t.count); \
+                PBF_KERNEL((KERNEL, false>), dim3(grid_), dim3(BS), 0, ctx->stream, __VA_ARGS__, t.count); \
         });                                                                                                    \
     } while (0)
 
@@ -932,11 +932,11 @@ hipError_t launch_friends(pbf_ctx* ctx, bool lambda0) {
     if (ctx->friends_staged) {
         PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS], {
             if (lambda0)
-                hipLaunchKernelGGL((k_friends<BS, true>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, xs,
+                PBF_KERNEL((k_friends<BS, true>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, xs,
                                    cells, ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->rho, ctx->lambda, ctx->n,
                                    fresh, ctx->err);
             else
-                hipLaunchKernelGGL((k_friends<BS, false>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, xs,
+                PBF_KERNEL((k_friends<BS, false>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, xs,
                                    cells, ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->rho, ctx->lambda, ctx->n,
                                    fresh, ctx->err);
         });
@@ -1003,6 +1003,15 @@ hipError_t launch_vorticity(pbf_ctx* ctx) {
     const BlockTable& t = table_for(ctx, F_VORT);
     PBF_LAUNCH_TILED(F_VORT, k_vorticity<BS, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->v, ctx->omega, ctx->nb_cnt,
                      ctx->nb_quads, ctx->n);
+    track_end(ctx);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(64) void k_tare(uint32_t) {}
+
+hipError_t launch_tare(pbf_ctx* ctx) {
+    track_begin(ctx, "eventTare");
+    PBF_KERNEL(k_tare, dim3(1), dim3(64), 0, ctx->stream, 0u);
     track_end(ctx);
     return hipGetLastError();
 }

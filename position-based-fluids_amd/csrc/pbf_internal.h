@@ -1,6 +1,7 @@
 // pbf_internal.h -- context layout and launcher declarations (not exported).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -72,6 +73,17 @@ struct GraphEntry {
 };
 
 }  // namespace pbf
+
+// hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHMEM, STREAM, args...) on a
+// context, with the current tracker's events when one is open.
+#define PBF_KERNEL(KERNEL, GRID, BLOCK, SHMEM, STREAM, ...)                                                  \
+    do {                                                                                                     \
+        if (ctx->launch_start)                                                                               \
+            hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, SHMEM, STREAM, ctx->launch_start, ctx->launch_stop, 0u, \
+                                  __VA_ARGS__);                                                              \
+        else                                                                                                 \
+            hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHMEM, STREAM, __VA_ARGS__);                             \
+    } while (0)
 
 #define HIP_TRY(ctx, call)                                                                                   \
     do {                                                                                                     \
@@ -151,6 +163,8 @@ struct pbf_ctx {
     std::vector<pbf::Tracker> trackers;
     std::vector<pbf::EventPair> events;  // recorded (start, stop) pairs
     size_t events_used = 0;
+    // the pair of the tracked launch being enqueued (null outside one)
+    hipEvent_t launch_start = nullptr, launch_stop = nullptr;
     std::string last_error;
 };
 
@@ -170,6 +184,9 @@ hipError_t launch_lambda(pbf_ctx* ctx, int iter, float4* aux = nullptr);
 hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last, float4* fut = nullptr);
 hipError_t launch_viscosity(pbf_ctx* ctx);
 hipError_t launch_vorticity(pbf_ctx* ctx);
+// profiling only: an empty kernel under its own tracker ("eventTare"), the
+// fixed part of a tracked launch's duration
+hipError_t launch_tare(pbf_ctx* ctx);
 
 // Records a formatted error on ctx (or for the next pbf_create when ctx is
 // NULL) and returns code.
@@ -183,7 +200,10 @@ SceneConsts make_consts(const pbf_params& P, const pbf_options& o, uint32_t cell
 // Enqueues one substep (Simulation::Step) on ctx->stream.
 int enqueue_step(pbf_ctx* ctx, uint32_t pause);
 
-// Profiling brackets (no-ops unless ctx->profiling).
+// Profiling brackets (no-ops unless ctx->profiling) around one kernel
+// launch, which PBF_KERNEL makes: the tracker's events are stamped by the
+// kernel itself at its start and end (hipExtLaunchKernelGGL), so a duration
+// excludes the dispatch of the launch, as a profiler's kernel trace does.
 void track_begin(pbf_ctx* ctx, const char* name, int index = -1);
 void track_end(pbf_ctx* ctx);
 void collect_timings(pbf_ctx* ctx);

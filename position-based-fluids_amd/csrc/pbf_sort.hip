@@ -182,7 +182,7 @@ hipError_t launch_sort(pbf_ctx* ctx) {
         const uint32_t* vin = (p == 0) ? nullptr : ((p & 1) ? ctx->vals_a : ctx->vals_b);
         uint32_t* vout = last ? ctx->perm : ((p & 1) ? ctx->vals_b : ctx->vals_a);
         track_begin(ctx, "onesweep", (int)p);
-        hipLaunchKernelGGL(k_onesweep, dim3(ctx->sort.tiles), dim3(kSortThreads), 0, ctx->stream, kin, vin, kout,
+        PBF_KERNEL(k_onesweep, dim3(ctx->sort.tiles), dim3(kSortThreads), 0, ctx->stream, kin, vin, kout,
                            vout, n, (uint32_t)(kRadixBits * p), ctx->sort.hist + (size_t)p * kRadix,
                            ctx->sort.status + (size_t)p * ctx->sort.tiles * kRadix, ctx->sort.ctr + p, ctx->err);
         track_end(ctx);

@@ -1,56 +1,80 @@
-// pbf_params.cpp -- the reference's .par loader (src/ParamUtils.cpp:24-100)
-// behind the C ABI: '#' starts a comment, keys are case-insensitive, one
-// "key value" pair per line, unknown keys are reported and skipped.
+// pbf_params.cpp -- the scenario (.par) format of the reference's loader
+// (src/ParamUtils.cpp:24-100) behind the C ABI: '#' starts a comment, keys
+// are case-insensitive, one "key value" pair per line, unknown keys are
+// counted and skipped, and a last line without a newline is not read.
 #include <algorithm>
 #include <cctype>
+#include <cstddef>
 #include <sstream>
 #include <string>
 
 #include "../../include/pbf/pbf.h"
 
+namespace {
+
+// One scenario key: its lower-case name, the field it sets and how the value
+// is read (stream extraction into the field's own type, so a value that does
+// not parse behaves as it does in the reference).
+struct ParamKey {
+    const char* name;
+    size_t offset;
+    void (*read)(std::istream&, unsigned char*);
+};
+
+template <typename T>
+void read_field(std::istream& in, unsigned char* field) {
+    in >> *reinterpret_cast<T*>(field);
+}
+
+#define PBF_KEY(NAME, FIELD)                                                                       \
+    ParamKey {                                                                                     \
+        NAME, offsetof(pbf_params, FIELD), &read_field<decltype(pbf_params::FIELD)>                \
+    }
+
+// The keys of src/ParamUtils.cpp:41-95 (the friends-circle sizes, h^2 and the
+// kernel factors are fixed or computed, not read).
+const ParamKey kKeys[] = {
+    PBF_KEY("resetsimonchange", resetSimOnChange), PBF_KEY("particlecount", particleCount),
+    PBF_KEY("xmin", xMin), PBF_KEY("xmax", xMax), PBF_KEY("ymin", yMin), PBF_KEY("ymax", yMax),
+    PBF_KEY("zmin", zMin), PBF_KEY("zmax", zMax), PBF_KEY("wavegenamp", waveGenAmp),
+    PBF_KEY("wavegenfreq", waveGenFreq), PBF_KEY("wavegenduty", waveGenDuty), PBF_KEY("timestep", timeStep),
+    PBF_KEY("simiterations", simIterations), PBF_KEY("substeps", subSteps), PBF_KEY("smoothlen", h),
+    PBF_KEY("gridbuffersize", gridBufSize), PBF_KEY("restdensity", restDensity), PBF_KEY("epsilon", epsilon),
+    PBF_KEY("garvity", garvity), PBF_KEY("vorticityfactor", vorticityFactor),
+    PBF_KEY("viscosityfactor", viscosityFactor), PBF_KEY("surfacetenstionk", surfaceTenstionK),
+    PBF_KEY("surfacetenstiondist", surfaceTenstionDist), PBF_KEY("setupspacing", setupSpacing),
+    PBF_KEY("segmentsize", segmentSize), PBF_KEY("sortiterations", sortIterations),
+    PBF_KEY("particlerendersize", particleRenderSize), PBF_KEY("enablecachedbuffers", EnableCachedBuffers),
+};
+
+#undef PBF_KEY
+
+const ParamKey* find_key(const std::string& name) {
+    for (const ParamKey& k : kKeys)
+        if (name == k.name) return &k;
+    return nullptr;
+}
+
+}  // namespace
+
 extern "C" int pbf_load_parameters(const char* text, pbf_params* p) {
     if (!text || !p) return PBF_E_INVALID;
-    std::istringstream ss{std::string(text)};
-    std::string line, key;
+    std::istringstream doc{std::string(text)};
+    unsigned char* const base = reinterpret_cast<unsigned char*>(p);
     int unknown = 0;
-    while (std::getline(ss, line)) {
-        // ParamUtils.cpp:35-39: a stream that is no longer good() (a last
-        // line without a trailing newline) ends the parse.
-        if (!ss.good()) break;
+    for (std::string line; std::getline(doc, line);) {
+        // the format's last-line rule: a line that ends the stream without a
+        // newline (the stream is no longer good()) ends the parse unread
+        if (!doc.good()) break;
         line.erase(std::find(line.begin(), line.end(), '#'), line.end());
-        if (line.empty()) continue;
-        std::transform(line.begin(), line.end(), line.begin(), [](unsigned char ch) { return (char)std::tolower(ch); });
-        std::istringstream ls(line);
-        if (!(ls >> key)) continue;
-        if (key == "resetsimonchange") ls >> p->resetSimOnChange;
-        else if (key == "particlecount") ls >> p->particleCount;
-        else if (key == "xmin") ls >> p->xMin;
-        else if (key == "xmax") ls >> p->xMax;
-        else if (key == "ymin") ls >> p->yMin;
-        else if (key == "ymax") ls >> p->yMax;
-        else if (key == "zmin") ls >> p->zMin;
-        else if (key == "zmax") ls >> p->zMax;
-        else if (key == "wavegenamp") ls >> p->waveGenAmp;
-        else if (key == "wavegenfreq") ls >> p->waveGenFreq;
-        else if (key == "wavegenduty") ls >> p->waveGenDuty;
-        else if (key == "timestep") ls >> p->timeStep;
-        else if (key == "simiterations") ls >> p->simIterations;
-        else if (key == "substeps") ls >> p->subSteps;
-        else if (key == "smoothlen") ls >> p->h;
-        else if (key == "gridbuffersize") ls >> p->gridBufSize;
-        else if (key == "restdensity") ls >> p->restDensity;
-        else if (key == "epsilon") ls >> p->epsilon;
-        else if (key == "garvity") ls >> p->garvity;
-        else if (key == "vorticityfactor") ls >> p->vorticityFactor;
-        else if (key == "viscosityfactor") ls >> p->viscosityFactor;
-        else if (key == "surfacetenstionk") ls >> p->surfaceTenstionK;
-        else if (key == "surfacetenstiondist") ls >> p->surfaceTenstionDist;
-        else if (key == "setupspacing") ls >> p->setupSpacing;
-        else if (key == "segmentsize") ls >> p->segmentSize;
-        else if (key == "sortiterations") ls >> p->sortIterations;
-        else if (key == "particlerendersize") ls >> p->particleRenderSize;
-        else if (key == "enablecachedbuffers") ls >> p->EnableCachedBuffers;
-        else ++unknown;
+        for (char& ch : line) ch = (char)std::tolower((unsigned char)ch);
+        std::istringstream fields(line);
+        std::string name;
+        if (!(fields >> name)) continue;   // blank or comment-only
+        if (const ParamKey* k = find_key(name))
+            k->read(fields, base + k->offset);
+        else
+            ++unknown;
     }
     pbf_params_finalize(p);
     return unknown;

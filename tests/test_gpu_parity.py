@@ -457,8 +457,10 @@ def test_profiling_trackers():
     t = sim.timings()
     # the first computeScaling runs inside buildFriendsList
     for name in ("predictPositions", "onesweep_0", "sortParticles", "buildFriendsList",
-                 "computeScaling_1", "computeDelta_3", "applyViscosity", "applyVorticity"):
+                 "computeScaling_1", "computeDelta_3", "applyViscosity", "applyVorticity", "eventTare"):
         assert name in t and t[name]["calls"] == 3 and t[name]["last_ms"] > 0.0, name
+    # the empty kernel's duration is the fixed part of every launch's
+    assert t["eventTare"]["sum_ms"] < t["buildFriendsList"]["sum_ms"]
     sim.close()
 
 

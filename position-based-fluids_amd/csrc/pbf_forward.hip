@@ -193,16 +193,18 @@ struct LambdaAcc {
 
 // computeScaling for particle i from its sweep list (packed circle counts
 // and quads): density, lambda, and lambda into x*.w in place (no thread of a
-// launch reads another particle's w).  TAPE also stores (sum of gradients,
+// launch reads another particle's w).  load(j) gathers x*_j (only its x, y,
+// z are used).  TAPE also stores (sum of gradients,
 // denominator) for the adjoint.
-template <bool TAPE>
+template <bool TAPE, class Load>
 __device__ __forceinline__ void scaling_of(uint32_t i, uint32_t packed, float4* __restrict__ xs,
                                            const uint4* __restrict__ quads, float* __restrict__ rho,
-                                           float* __restrict__ lambda, float4* __restrict__ aux, const SceneConsts& c) {
+                                           float* __restrict__ lambda, float4* __restrict__ aux, const SceneConsts& c,
+                                           Load&& load) {
     const float4 p = xs[i];
     LambdaAcc a;
     walk_friends<3, 5, true>(
-        i, packed, quads, [&](uint32_t j) { return xs[j]; },
+        i, packed, quads, load,
         [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
     const float dens = a.dens * c.poly6;
     rho[i] = dens;
@@ -470,7 +472,8 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
                             scan_bucket_global(p, i, cells[nh(dx, dy, dz, c)], n, xs, c, list, quads, w);
             w.finish(i, nb_cnt, list, quads, err);
             pack_friends(i, w.packed, list, quads);
-            if (LAMBDA0) scaling_of<false>(i, w.packed, xs, quads, rho, lambda, nullptr, c);
+            if (LAMBDA0) scaling_of<false>(i, w.packed, xs, quads, rho, lambda, nullptr, c,
+                                         [&](uint32_t j) { return xs[j]; });
         }
         if (!STRIDED) break;   // one workgroup per tile: a single pass
     }
@@ -611,7 +614,8 @@ __global__ __launch_bounds__(BS) void k_friends(const SceneConsts* __restrict__ 
             }
     w.finish(i, nb_cnt, list, quads, err);
     pack_friends(i, w.packed, list, quads);
-    if (LAMBDA0) scaling_of<false>(i, w.packed, xs, quads, rho, lambda, nullptr, c);
+    if (LAMBDA0) scaling_of<false>(i, w.packed, xs, quads, rho, lambda, nullptr, c,
+                                         [&](uint32_t j) { return xs[j]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -626,6 +630,27 @@ struct Pair4 {
     float4 a, b;
 };
 
+// A tile's neighbourhood in LDS.  In the cell-sorted (Morton) order about
+// four in five friends of the particles of a 256-particle tile lie within
+// kNearPad entries of the tile; a sweep copies that window of the arrays it
+// gathers into LDS, and a gather inside the window reads LDS instead of going
+// through the texture-address and L1 path.  The values are the same either
+// way.
+constexpr uint32_t kNearPad = 256;
+template <int BS>
+struct NearWindow {
+    static constexpr uint32_t kLen = BS + 2 * kNearPad;
+    uint32_t base = 0, len = 0;
+    // entries [base, base + len) of the tile be (the array holds n)
+    __device__ __forceinline__ NearWindow(const uint4& be, uint32_t n) {
+        base = be.y > kNearPad ? be.y - kNearPad : 0u;
+        len = min(kLen, n - base);
+    }
+    __device__ __forceinline__ void stage(float4* __restrict__ dst, const float4* __restrict__ src) const {
+        for (uint32_t t = threadIdx.x; t < len; t += BS) dst[t] = src[base + t];
+    }
+};
+
 // TAPE also stores (sum of gradients, denominator) of every particle for the
 // adjoint (aux).
 template <int BS, bool TAPE, bool STRIDED>
@@ -634,12 +659,22 @@ __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ s
                                               const uint4* __restrict__ nb_list, float* __restrict__ rho,
                                               float* __restrict__ lambda, float4* __restrict__ aux, uint32_t n,
                                               uint32_t ntiles) {
+    __shared__ float4 s_x[NearWindow<BS>::kLen];
     for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
+        const NearWindow<BS> near(be, n);
+        near.stage(s_x, xs);   // w is rewritten below, but only x, y, z are gathered
+        __syncthreads();
         const uint32_t i = be.y + threadIdx.x;
-        if (i < be.z) scaling_of<TAPE>(i, nb_cnt[i], xs, nb_list, rho, lambda, aux, c);
+        if (i < be.z)
+            scaling_of<TAPE>(i, nb_cnt[i], xs, nb_list, rho, lambda, aux, c, [&](uint32_t j) {
+                const uint32_t o = j - near.base;
+                if (o < near.len) return s_x[o];
+                return xs[j];
+            });
         if (!STRIDED) break;   // one workgroup per tile: a single pass
+        __syncthreads();       // the window of the next tile
     }
 }
 
@@ -675,15 +710,24 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
                                              const float* __restrict__ rho, const uint32_t* __restrict__ nb_cnt,
                                              const uint4* __restrict__ nb_list, float4* __restrict__ fut, uint32_t n,
                                              uint32_t ntiles) {
+    __shared__ float4 s_x[NearWindow<BS>::kLen];
     for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
+        const NearWindow<BS> near(be, n);
+        near.stage(s_x, xs);
+        __syncthreads();
         const uint32_t i = be.y + threadIdx.x;
         if (i < be.z) {
             const float4 p = xs[i];
             DeltaAcc a;
             walk_friends<1, 2, true>(
-                i, nb_cnt[i], nb_list, [&](uint32_t j) { return xs[j]; },
+                i, nb_cnt[i], nb_list,
+                [&](uint32_t j) {
+                    const uint32_t o = j - near.base;
+                    if (o < near.len) return s_x[o];
+                    return xs[j];
+                },
                 [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
             float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
             float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
@@ -713,6 +757,7 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
             }
         }
         if (!STRIDED) break;   // one workgroup per tile: a single pass
+        __syncthreads();       // the window of the next tile
     }
 }
 
@@ -747,16 +792,26 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
                                                  const float4* __restrict__ v_in, float4* __restrict__ v_out,
                                                  float4* __restrict__ omega, const uint32_t* __restrict__ nb_cnt,
                                                  const uint4* __restrict__ nb_list, uint32_t n, uint32_t ntiles) {
+    __shared__ float4 s_x[NearWindow<BS>::kLen], s_v[NearWindow<BS>::kLen];
     for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
+        const NearWindow<BS> near(be, n);
+        near.stage(s_x, xs);
+        near.stage(s_v, v_in);
+        __syncthreads();
         const uint32_t i = be.y + threadIdx.x;
         if (i < be.z) {
             const float4 p = xs[i];
             const float4 vi = v_in[i];
             ViscAcc a;
             walk_friends<1, 2, true>(
-                i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], v_in[j]}; },
+                i, nb_cnt[i], nb_list,
+                [&](uint32_t j) {
+                    const uint32_t o = j - near.base;
+                    if (o < near.len) return Pair4{s_x[o], s_v[o]};
+                    return Pair4{xs[j], v_in[j]};
+                },
                 [&](const Pair4& t, bool valid) { a.add(p, vi, t.a, t.b, c, valid); });
             float4 vo;
             vo.x = vi.x + c.viscosity * (a.vsx * c.poly6);
@@ -771,6 +826,7 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
             omega[i] = make_float4(ox, oy, oz, norm_of_sq(ox * ox + oy * oy + oz * oz));
         }
         if (!STRIDED) break;   // one workgroup per tile: a single pass
+        __syncthreads();       // the window of the next tile
     }
 }
 
@@ -799,15 +855,25 @@ __global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict_
                                                  float4* __restrict__ v, const float4* __restrict__ omega,
                                                  const uint32_t* __restrict__ nb_cnt,
                                                  const uint4* __restrict__ nb_list, uint32_t n, uint32_t ntiles) {
+    __shared__ float4 s_x[NearWindow<BS>::kLen], s_o[NearWindow<BS>::kLen];
     for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
+        const NearWindow<BS> near(be, n);
+        near.stage(s_x, xs);
+        near.stage(s_o, omega);
+        __syncthreads();
         const uint32_t i = be.y + threadIdx.x;
         if (i < be.z) {
             const float4 p = xs[i];
             VortAcc a;
             walk_friends<1, 2, true>(
-                i, nb_cnt[i], nb_list, [&](uint32_t j) { return Pair4{xs[j], omega[j]}; },
+                i, nb_cnt[i], nb_list,
+                [&](uint32_t j) {
+                    const uint32_t o = j - near.base;
+                    if (o < near.len) return Pair4{s_x[o], s_o[o]};
+                    return Pair4{xs[j], omega[j]};
+                },
                 [&](const Pair4& t, bool valid) { a.add(p, t.a, t.b, c, valid); });
             float ex = a.ex * -c.spiky, ey = a.ey * -c.spiky, ez = a.ez * -c.spiky;
             const float l = sqrtf(ex * ex + ey * ey + ez * ez);
@@ -823,6 +889,7 @@ __global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict_
             v[i] = vi;
         }
         if (!STRIDED) break;   // one workgroup per tile: a single pass
+        __syncthreads();       // the window of the next tile
     }
 }
 

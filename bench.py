@@ -48,18 +48,19 @@ STAGE_BYTES = {
 # kernel sources whose hash is stamped here (KERNEL_SOURCES): with any other
 # source the line reports traffic as null.
 PMC_TRAFFIC = {
-    "source_sha16": "0a5fb80c1dd5cda1",
+    "source_sha16": "c962af6ae1491d69",
     "bytes_per_launch": {
-        "applyViscosity": 217137251,
-        "applyVorticity": 212663212,
-        "buildFriendsList": 921071385,
-        "computeDelta": 199333238,
-        "computeScaling": 198288699,
-        "onesweep": 17344987,
-        "predictPositions": 43911365,
-        "sortParticles": 96392355,
+        "applyViscosity": 223774563,
+        "applyVorticity": 229816143,
+        "buildFriendsList": 923022121,
+        "computeDelta": 205309267,
+        "computeScaling": 202276427,
+        "onesweep": 17288309,
+        "predictPositions": 43909799,
+        "sortParticles": 96384372,
     },
 }
+
 KERNEL_SOURCES = ("pbf_forward.hip", "pbf_device.h", "pbf_sort.hip", "pbf_internal.h")
 
 

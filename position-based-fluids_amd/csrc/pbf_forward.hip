@@ -633,9 +633,11 @@ struct Pair4 {
 // A tile's neighbourhood in LDS.  In the cell-sorted (Morton) order about
 // four in five friends of the particles of a 256-particle tile lie within
 // kNearPad entries of the tile; a sweep copies that window of the arrays it
-// gathers into LDS, and a gather inside the window reads LDS instead of going
-// through the texture-address and L1 path.  The values are the same either
-// way.
+// gathers into LDS, and a gather inside the window reads LDS instead of the
+// L1 cache.  The values are the same either way.  The compiler turns "LDS
+// inside the window, global outside" into one generic (flat) load through a
+// selected address, which serves a wave's mixed lanes in one instruction; an
+// LDS read followed by an exec-masked global read measured 10-40% slower.
 constexpr uint32_t kNearPad = 256;
 template <int BS>
 struct NearWindow {

@@ -419,17 +419,22 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b1(const SceneConsts* __re
                                                         const uint4* __restrict__ quads,
                                                         const float4* __restrict__ gv, const float* __restrict__ scale,
                                                         float4* __restrict__ z, float4* __restrict__ gom,
-                                                        float* __restrict__ part) {
+                                                        float* __restrict__ part, uint32_t n, uint32_t near_on) {
+    __shared__ float4 s_P[NearWindow<kAdjThreads>::kLen], s_om[NearWindow<kAdjThreads>::kLen];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint4 be = blocks[tile];
     const SceneConsts& c = scenes[be.x];
+    const NearWindow<kAdjThreads> near(be, n, near_on);   // the tile's neighbourhood in LDS
+    near.stage(s_P, P);
+    near.stage(s_om, om);
+    __syncthreads();
     const uint32_t i = be.y + threadIdx.x;
     float ge[1] = {0.0f};
     if (i < be.z) {
         const float4 p = P[i];
         V3 eta = {0.f, 0.f, 0.f};
         walk_friends<1, 2>(
-            i, cnt[i], quads, [&](uint32_t j) { return G2{P[j], om[j]}; },
+            i, cnt[i], quads, [&](uint32_t j) { return G2{near.get(s_P, P, j), near.get(s_om, om, j)}; },
             [&](const G2& t) {
                 const Pair g = pair_of(p, t.a, c);
                 const V3 oj = v3(t.b);
@@ -457,16 +462,21 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b2(
     const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ P,
     const float4* __restrict__ om, const uint32_t* __restrict__ cnt, const uint4* __restrict__ quads,
     const uint32_t* __restrict__ tcnt, const uint4* __restrict__ tquads, const float4* __restrict__ z,
-    const float* __restrict__ scale, float4* __restrict__ gom, float4* __restrict__ gx) {
+    const float* __restrict__ scale, float4* __restrict__ gom, float4* __restrict__ gx, uint32_t n, uint32_t near_on) {
+    __shared__ float4 s_P[NearWindow<kAdjThreads>::kLen], s_om[NearWindow<kAdjThreads>::kLen];
     const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
     const SceneConsts& c = scenes[be.x];
+    const NearWindow<kAdjThreads> near(be, n, near_on);   // the tile's neighbourhood in LDS
+    near.stage(s_P, P);
+    near.stage(s_om, om);
+    __syncthreads();
     const uint32_t m = be.y + threadIdx.x;
     if (m >= be.z) return;
     const float4 p = P[m];
     const V3 zm = v3(z[m]);
     V3 gP = {0.f, 0.f, 0.f};
     walk_friends<1, 2>(
-        m, cnt[m], quads, [&](uint32_t j) { return G2{P[j], om[j]}; },
+        m, cnt[m], quads, [&](uint32_t j) { return G2{near.get(s_P, P, j), near.get(s_om, om, j)}; },
         [&](const G2& t) {
             const Pair g = pair_of(p, t.a, c);
             const V3 oj = v3(t.b);
@@ -477,7 +487,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_vort_b2(
         const float wl = norm_of_sq(dot(omm, omm));
         float gs = 0.0f;
         walk_quads(
-            m, t05(tcnt[m]), tquads, [&](uint32_t i) { return G2{P[i], z[i]}; },
+            m, t05(tcnt[m]), tquads, [&](uint32_t i) { return G2{near.get(s_P, P, i), z[i]}; },
             [&](const G2& t) {
                 const Pair g = pair_of(t.a, p, c);
                 const V3 zi = v3(t.b);
@@ -497,10 +507,16 @@ __global__ __launch_bounds__(kAdjThreads) void k_xsph_b(
     const float4* __restrict__ vu, const uint32_t* __restrict__ cnt, const uint4* __restrict__ quads,
     const uint32_t* __restrict__ tcnt, const uint4* __restrict__ tquads, const float4* __restrict__ gv,
     const float* __restrict__ scale, const float4* __restrict__ gom, float4* __restrict__ gx,
-    float4* __restrict__ gxs, float4* __restrict__ z, const float4* __restrict__ fut, float* __restrict__ part) {
+    float4* __restrict__ gxs, float4* __restrict__ z, const float4* __restrict__ fut, float* __restrict__ part,
+    uint32_t n, uint32_t near_on) {
+    __shared__ float4 s_P[NearWindow<kAdjThreads>::kLen], s_vu[NearWindow<kAdjThreads>::kLen];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint4 be = blocks[tile];
     const SceneConsts& c = scenes[be.x];
+    const NearWindow<kAdjThreads> near(be, n, near_on);   // the tile's neighbourhood in LDS
+    near.stage(s_P, P);
+    near.stage(s_vu, vu);
+    __syncthreads();
     const uint32_t m = be.y + threadIdx.x;
     float gc[1] = {0.0f};
     if (m < be.z) {
@@ -513,7 +529,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_xsph_b(
         const V3 gam = v3(gom[m]) * -c.spiky;
         V3 S = {0.f, 0.f, 0.f}, gVu = {0.f, 0.f, 0.f}, gP = {0.f, 0.f, 0.f};
         walk_friends<1, 2>(
-            m, cnt[m], quads, [&](uint32_t j) { return G2{P[j], vu[j]}; },
+            m, cnt[m], quads, [&](uint32_t j) { return G2{near.get(s_P, P, j), near.get(s_vu, vu, j)}; },
             [&](const G2& t) {
                 const Pair g = pair_of(p, t.a, c);
                 const bool in = g.in && fabsf(t.a.w) > 1e-8f;
@@ -526,7 +542,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_xsph_b(
         gc[0] = dot(S * c.poly6, gvm);
         if (fabsf(p.w) > 1e-8f) {
             walk_quads(
-                m, t05(tcnt[m]), tquads, [&](uint32_t i) { return G4{P[i], vu[i], gv[i], gom[i]}; },
+                m, t05(tcnt[m]), tquads,
+                [&](uint32_t i) { return G4{near.get(s_P, P, i), near.get(s_vu, vu, i), gv[i], gom[i]}; },
                 [&](const G4& t) {
                     const Pair g = pair_of(t.a, p, c);
                     const V3 u = vm - v3(t.b);
@@ -555,9 +572,14 @@ __global__ __launch_bounds__(kAdjThreads) void k_delta_b(
     const SceneConsts* __restrict__ scenes, const uint4* __restrict__ blocks, const float4* __restrict__ P,
     const float4* __restrict__ aux, const uint32_t* __restrict__ cnt, const uint4* __restrict__ quads,
     const uint32_t* __restrict__ tcnt, const uint4* __restrict__ tquads, const float4* __restrict__ z,
-    float4* __restrict__ gx, float4* __restrict__ z2, float* __restrict__ zb) {
+    float4* __restrict__ gx, float4* __restrict__ z2, float* __restrict__ zb, uint32_t n, uint32_t near_on) {
+    __shared__ float4 s_P[NearWindow<kAdjThreads>::kLen], s_z[NearWindow<kAdjThreads>::kLen];
     const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];
     const SceneConsts& c = scenes[be.x];
+    const NearWindow<kAdjThreads> near(be, n, near_on);   // the tile's neighbourhood in LDS
+    near.stage(s_P, P);
+    near.stage(s_z, z);
+    __syncthreads();
     const uint32_t m = be.y + threadIdx.x;
     if (m >= be.z) return;
     const float4 p = P[m];
@@ -566,7 +588,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_delta_b(
     float gl = 0.0f;
     V3 gP = {0.f, 0.f, 0.f};
     walk_friends<1, 2>(
-        m, cnt[m], quads, [&](uint32_t j) { return P[j]; },
+        m, cnt[m], quads, [&](uint32_t j) { return near.get(s_P, P, j); },
         [&](const float4& pj) {
             const Pair g = pair_of(p, pj, c);
             const V3 ps = psi(g);
@@ -578,7 +600,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_delta_b(
             gP += sel(g.in, jpsi(g, km, c) * coef + g.r * (t * ks * r3 * g.d * g.d));
         });
     walk_quads(
-        m, t05(tcnt[m]), tquads, [&](uint32_t i) { return G2{P[i], z[i]}; },
+        m, t05(tcnt[m]), tquads, [&](uint32_t i) { return G2{near.get(s_P, P, i), near.get(s_z, z, i)}; },
         [&](const G2& t) {
             const Pair g = pair_of(t.a, p, c);
             const V3 ki = v3(t.b);
@@ -613,10 +635,15 @@ __global__ __launch_bounds__(kAdjThreads) void k_lambda_b(
     const uint4* __restrict__ tquads, const float4* __restrict__ z2, const float* __restrict__ zb,
     float4* __restrict__ gx, const float4* __restrict__ fut_prev, float4* __restrict__ z,
     const uint32_t* __restrict__ perm, const float4* __restrict__ gxs, float4* __restrict__ gx_out,
-    float4* __restrict__ gv_out, float* __restrict__ part, uint32_t n) {
+    float4* __restrict__ gv_out, float* __restrict__ part, uint32_t n, uint32_t near_on) {
+    __shared__ float4 s_P[NearWindow<kAdjThreads>::kLen], s_z2[NearWindow<kAdjThreads>::kLen];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint4 be = blocks[tile];
     const SceneConsts& c = scenes[be.x];
+    const NearWindow<kAdjThreads> near(be, n, near_on);   // the tile's neighbourhood in LDS
+    near.stage(s_P, P);
+    near.stage(s_z2, z2);
+    __syncthreads();
     const uint32_t m = be.y + threadIdx.x;
     float gg[3] = {0.0f, 0.0f, 0.0f};
     float mx = 0.0f;
@@ -627,14 +654,15 @@ __global__ __launch_bounds__(kAdjThreads) void k_lambda_b(
         const float Am = qm.w, Bm = zb[m];
         V3 gP = {0.f, 0.f, 0.f};
         walk_friends<3, 5>(
-            m, cnt[m], quads, [&](uint32_t j) { return P[j]; },
+            m, cnt[m], quads, [&](uint32_t j) { return near.get(s_P, P, j); },
             [&](const float4& pj) {
                 const Pair g = pair_of(p, pj, c);
                 const V3 ps = psi(g);
                 gP += sel(g.in, g.r * (6.0f * Am * g.d * g.d) + jpsi(g, ps * (Bm * c.spiky) + Hm, c));
             });
         walk_quads(
-            m, t06(tcnt[m]), tquads, [&](uint32_t i) { return G2f{P[i], z2[i], zb[i]}; },
+            m, t06(tcnt[m]), tquads,
+            [&](uint32_t i) { return G2f{near.get(s_P, P, i), near.get(s_z2, z2, i), zb[i]}; },
             [&](const G2f& t) {
                 const Pair g = pair_of(t.a, p, c);
                 const V3 ps = psi(g);
@@ -957,24 +985,27 @@ int backward_substep(pbf_adjoint* a, const TapeSlot& t) {
     const dim3 grid(bt.count), block(kAdjThreads);
     hipStream_t st = ctx->stream;
     const float4* PK = t.P + (size_t)K * n;
+    // the tiles' neighbourhoods in LDS on grids of kNearMinTilesPerCu tiles per CU or more
+    const uint32_t near = bt.count >= kNearMinTilesPerCu * (uint32_t)std::max(ctx->cu_count, 1) ? 1u : 0u;
     hipLaunchKernelGGL(k_vort_b1, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.om, t.cnt, t.quads, a->gv, a->scale,
-                       a->z, a->gom, a->part);
+                       a->z, a->gom, a->part, ctx->n, near);
     hipLaunchKernelGGL(k_vort_b2, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.om, t.cnt, t.quads, t.tcnt, t.tquads,
-                       a->z, a->scale, a->gom, a->gx);
+                       a->z, a->scale, a->gom, a->gx, ctx->n, near);
     hipLaunchKernelGGL(k_xsph_b, grid, block, 0, st, ctx->sc_d, bt.d, PK, t.vu, t.cnt, t.quads, t.tcnt, t.tquads,
-                       a->gv, a->scale, a->gom, a->gx, a->gxs, a->z, t.fut + (size_t)(K - 1) * n, a->part);
+                       a->gv, a->scale, a->gom, a->gx, a->gxs, a->z, t.fut + (size_t)(K - 1) * n, a->part, ctx->n,
+                       near);
     for (int k = K - 1; k >= 0; --k) {
         const float4* Pk = t.P + (size_t)k * n;
         hipLaunchKernelGGL(k_delta_b, grid, block, 0, st, ctx->sc_d, bt.d, Pk, t.aux + (size_t)k * n, t.cnt,
-                           t.quads, t.tcnt, t.tquads, a->z, a->gx, a->z2, a->zb);
+                           t.quads, t.tcnt, t.tquads, a->z, a->gx, a->z2, a->zb, ctx->n, near);
         if (k > 0) {
             hipLaunchKernelGGL(k_lambda_b<false>, grid, block, 0, st, ctx->sc_d, bt.d, Pk, t.cnt, t.quads, t.tcnt,
                                t.tquads, a->z2, a->zb, a->gx, t.fut + (size_t)(k - 1) * n, a->z, t.perm, a->gxs,
-                               a->gx2, a->gv2, a->part, ctx->n);
+                               a->gx2, a->gv2, a->part, ctx->n, near);
         } else {
             hipLaunchKernelGGL(k_lambda_b<true>, grid, block, 0, st, ctx->sc_d, bt.d, Pk, t.cnt, t.quads, t.tcnt,
                                t.tquads, a->z2, a->zb, a->gx, nullptr, a->z, t.perm, a->gxs, a->gx2, a->gv2,
-                               a->part, ctx->n);
+                               a->part, ctx->n, near);
         }
     }
     hipLaunchKernelGGL(k_param_reduce, dim3(ctx->S), dim3(64), 0, st, a->tile_range, a->part, a->gparam, a->scale,
@@ -1265,7 +1296,8 @@ int pbf_adjoint_overflow(pbf_adjoint* a, uint32_t* scenes, uint32_t* per_scene) 
 int pbf_adjoint_scaled_gradients(pbf_adjoint* a, float* gx0, float* gv0, float* gcoef, int32_t* expo) {
     if (!a) return PBF_E_INVALID;
     pbf_ctx* ctx = a->ctx;
-    if (!a->have_backward) return pbf::fail(ctx, PBF_E_STATE, "pbf_adjoint_scaled_gradients before pbf_adjoint_backward");
+    if (!a->have_backward)
+        return pbf::fail(ctx, PBF_E_STATE, "pbf_adjoint_scaled_gradients before pbf_adjoint_backward");
     if (ctx->n == 0) return PBF_OK;
     if (gx0 || gv0) {
         const pbf::BlockTable& bt = ctx->bt[2];

@@ -630,29 +630,6 @@ struct Pair4 {
     float4 a, b;
 };
 
-// A tile's neighbourhood in LDS.  In the cell-sorted (Morton) order about
-// four in five friends of the particles of a 256-particle tile lie within
-// kNearPad entries of the tile; a sweep copies that window of the arrays it
-// gathers into LDS, and a gather inside the window reads LDS instead of the
-// L1 cache.  The values are the same either way.  The compiler turns "LDS
-// inside the window, global outside" into one generic (flat) load through a
-// selected address, which serves a wave's mixed lanes in one instruction; an
-// LDS read followed by an exec-masked global read measured 10-40% slower.
-constexpr uint32_t kNearPad = 256;
-template <int BS>
-struct NearWindow {
-    static constexpr uint32_t kLen = BS + 2 * kNearPad;
-    uint32_t base = 0, len = 0;
-    // entries [base, base + len) of the tile be (the array holds n)
-    __device__ __forceinline__ NearWindow(const uint4& be, uint32_t n) {
-        base = be.y > kNearPad ? be.y - kNearPad : 0u;
-        len = min(kLen, n - base);
-    }
-    __device__ __forceinline__ void stage(float4* __restrict__ dst, const float4* __restrict__ src) const {
-        for (uint32_t t = threadIdx.x; t < len; t += BS) dst[t] = src[base + t];
-    }
-};
-
 // TAPE also stores (sum of gradients, denominator) of every particle for the
 // adjoint (aux).
 template <int BS, bool TAPE, bool STRIDED>
@@ -660,21 +637,18 @@ __global__ __launch_bounds__(BS) void k_lambda(const SceneConsts* __restrict__ s
                                               float4* __restrict__ xs, const uint32_t* __restrict__ nb_cnt,
                                               const uint4* __restrict__ nb_list, float* __restrict__ rho,
                                               float* __restrict__ lambda, float4* __restrict__ aux, uint32_t n,
-                                              uint32_t ntiles) {
+                                              uint32_t near_on, uint32_t ntiles) {
     __shared__ float4 s_x[NearWindow<BS>::kLen];
     for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
-        const NearWindow<BS> near(be, n);
+        const NearWindow<BS> near(be, n, near_on);
         near.stage(s_x, xs);   // w is rewritten below, but only x, y, z are gathered
         __syncthreads();
         const uint32_t i = be.y + threadIdx.x;
         if (i < be.z)
-            scaling_of<TAPE>(i, nb_cnt[i], xs, nb_list, rho, lambda, aux, c, [&](uint32_t j) {
-                const uint32_t o = j - near.base;
-                if (o < near.len) return s_x[o];
-                return xs[j];
-            });
+            scaling_of<TAPE>(i, nb_cnt[i], xs, nb_list, rho, lambda, aux, c,
+                             [&](uint32_t j) { return near.get(s_x, xs, j); });
         if (!STRIDED) break;   // one workgroup per tile: a single pass
         __syncthreads();       // the window of the next tile
     }
@@ -711,12 +685,12 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
                                              float4* __restrict__ x, float4* __restrict__ v,
                                              const float* __restrict__ rho, const uint32_t* __restrict__ nb_cnt,
                                              const uint4* __restrict__ nb_list, float4* __restrict__ fut, uint32_t n,
-                                             uint32_t ntiles) {
+                                             uint32_t near_on, uint32_t ntiles) {
     __shared__ float4 s_x[NearWindow<BS>::kLen];
     for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
-        const NearWindow<BS> near(be, n);
+        const NearWindow<BS> near(be, n, near_on);
         near.stage(s_x, xs);
         __syncthreads();
         const uint32_t i = be.y + threadIdx.x;
@@ -724,12 +698,7 @@ __global__ __launch_bounds__(BS) void k_delta(const SceneConsts* __restrict__ sc
             const float4 p = xs[i];
             DeltaAcc a;
             walk_friends<1, 2, true>(
-                i, nb_cnt[i], nb_list,
-                [&](uint32_t j) {
-                    const uint32_t o = j - near.base;
-                    if (o < near.len) return s_x[o];
-                    return xs[j];
-                },
+                i, nb_cnt[i], nb_list, [&](uint32_t j) { return near.get(s_x, xs, j); },
                 [&](const float4& q, bool valid) { a.add(p, q, c, valid); });
             float fx = p.x + (-c.spiky * a.sx) / c.rest_density;
             float fy = p.y + (-c.spiky * a.sy) / c.rest_density;
@@ -793,12 +762,13 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
                                                  const uint4* __restrict__ blocks, const float4* __restrict__ xs,
                                                  const float4* __restrict__ v_in, float4* __restrict__ v_out,
                                                  float4* __restrict__ omega, const uint32_t* __restrict__ nb_cnt,
-                                                 const uint4* __restrict__ nb_list, uint32_t n, uint32_t ntiles) {
+                                                 const uint4* __restrict__ nb_list, uint32_t n, uint32_t near_on,
+                                                 uint32_t ntiles) {
     __shared__ float4 s_x[NearWindow<BS>::kLen], s_v[NearWindow<BS>::kLen];
     for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
-        const NearWindow<BS> near(be, n);
+        const NearWindow<BS> near(be, n, near_on);
         near.stage(s_x, xs);
         near.stage(s_v, v_in);
         __syncthreads();
@@ -809,11 +779,7 @@ __global__ __launch_bounds__(BS) void k_viscosity(const SceneConsts* __restrict_
             ViscAcc a;
             walk_friends<1, 2, true>(
                 i, nb_cnt[i], nb_list,
-                [&](uint32_t j) {
-                    const uint32_t o = j - near.base;
-                    if (o < near.len) return Pair4{s_x[o], s_v[o]};
-                    return Pair4{xs[j], v_in[j]};
-                },
+                [&](uint32_t j) { return Pair4{near.get(s_x, xs, j), near.get(s_v, v_in, j)}; },
                 [&](const Pair4& t, bool valid) { a.add(p, vi, t.a, t.b, c, valid); });
             float4 vo;
             vo.x = vi.x + c.viscosity * (a.vsx * c.poly6);
@@ -856,12 +822,13 @@ __global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict_
                                                  const uint4* __restrict__ blocks, const float4* __restrict__ xs,
                                                  float4* __restrict__ v, const float4* __restrict__ omega,
                                                  const uint32_t* __restrict__ nb_cnt,
-                                                 const uint4* __restrict__ nb_list, uint32_t n, uint32_t ntiles) {
+                                                 const uint4* __restrict__ nb_list, uint32_t n, uint32_t near_on,
+                                                 uint32_t ntiles) {
     __shared__ float4 s_x[NearWindow<BS>::kLen], s_o[NearWindow<BS>::kLen];
     for (uint32_t tv = blockIdx.x; tv < ntiles; tv += gridDim.x) {   // tiles b, b + grid, ...
         const uint4 be = blocks[xcd_tile(tv, ntiles)];
         const SceneConsts& c = scenes[be.x];
-        const NearWindow<BS> near(be, n);
+        const NearWindow<BS> near(be, n, near_on);
         near.stage(s_x, xs);
         near.stage(s_o, omega);
         __syncthreads();
@@ -871,11 +838,7 @@ __global__ __launch_bounds__(BS) void k_vorticity(const SceneConsts* __restrict_
             VortAcc a;
             walk_friends<1, 2, true>(
                 i, nb_cnt[i], nb_list,
-                [&](uint32_t j) {
-                    const uint32_t o = j - near.base;
-                    if (o < near.len) return Pair4{s_x[o], s_o[o]};
-                    return Pair4{xs[j], omega[j]};
-                },
+                [&](uint32_t j) { return Pair4{near.get(s_x, xs, j), near.get(s_o, omega, j)}; },
                 [&](const Pair4& t, bool valid) { a.add(p, t.a, t.b, c, valid); });
             float ex = a.ex * -c.spiky, ey = a.ey * -c.spiky, ez = a.ez * -c.spiky;
             const float l = sqrtf(ex * ex + ey * ey + ez * ez);
@@ -940,6 +903,12 @@ hipError_t query_residency(pbf_ctx* ctx) {
         });
     }
     return hipSuccess;   // a failed occupancy query falls back to one workgroup per CU
+}
+
+// Whether the sweeps copy their tiles' neighbourhoods into LDS (NearWindow):
+// only on grids of kNearMinTilesPerCu tiles per CU or more.
+static uint32_t near_on(const pbf_ctx* ctx, uint32_t tiles) {
+    return tiles >= kNearMinTilesPerCu * (uint32_t)std::max(ctx->cu_count, 1) ? 1u : 0u;
 }
 
 hipError_t launch_predict_hash(pbf_ctx* ctx, bool reset_old) {
@@ -1025,10 +994,10 @@ hipError_t launch_lambda(pbf_ctx* ctx, int iter, float4* aux) {
     const BlockTable& t = table_for(ctx, F_LAMBDA);
     if (aux)
         PBF_LAUNCH_TILED(F_LAMBDA, k_lambda<BS COMMA true, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt,
-                         ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n);
+                         ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n, near_on(ctx, t.count));
     else
         PBF_LAUNCH_TILED(F_LAMBDA, k_lambda<BS COMMA false, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->nb_cnt,
-                         ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n);
+                         ctx->nb_quads, ctx->rho, ctx->lambda, aux, ctx->n, near_on(ctx, t.count));
     track_end(ctx);
     return hipGetLastError();
 }
@@ -1039,7 +1008,7 @@ hipError_t launch_delta(pbf_ctx* ctx, int iter, bool last, float4* fut) {
     const BlockTable& t = table_for(ctx, F_DELTA);
 #define PBF_LAUNCH_DELTA(LASTV, TAPEV)                                                                           \
     PBF_LAUNCH_TILED(F_DELTA, k_delta<BS COMMA LASTV COMMA TAPEV, ctx->sc_d, t.d, ctx->xs[cur], ctx->xs[1 - cur], \
-                     ctx->x, ctx->v, ctx->rho, ctx->nb_cnt, ctx->nb_quads, fut, ctx->n)
+                     ctx->x, ctx->v, ctx->rho, ctx->nb_cnt, ctx->nb_quads, fut, ctx->n, near_on(ctx, t.count))
     if (last && fut) {
         PBF_LAUNCH_DELTA(true, true);
     } else if (last) {
@@ -1059,7 +1028,7 @@ hipError_t launch_viscosity(pbf_ctx* ctx) {
     track_begin(ctx, "applyViscosity");
     const BlockTable& t = table_for(ctx, F_VISC);
     PBF_LAUNCH_TILED(F_VISC, k_viscosity<BS, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->v, ctx->v_alt, ctx->omega,
-                     ctx->nb_cnt, ctx->nb_quads, ctx->n);
+                     ctx->nb_cnt, ctx->nb_quads, ctx->n, near_on(ctx, t.count));
     track_end(ctx);
     float4* tmp = ctx->v;
     ctx->v = ctx->v_alt;
@@ -1071,7 +1040,7 @@ hipError_t launch_vorticity(pbf_ctx* ctx) {
     track_begin(ctx, "applyVorticity");
     const BlockTable& t = table_for(ctx, F_VORT);
     PBF_LAUNCH_TILED(F_VORT, k_vorticity<BS, ctx->sc_d, t.d, ctx->xs[ctx->xs_cur], ctx->v, ctx->omega, ctx->nb_cnt,
-                     ctx->nb_quads, ctx->n);
+                     ctx->nb_quads, ctx->n, near_on(ctx, t.count));
     track_end(ctx);
     return hipGetLastError();
 }

@@ -139,6 +139,9 @@ extern "C" int pbf_run_noalloc(const pbf_params* params, const pbf_options* opt,
     c.total_cells = params->gridBufSize;
     c.device = dev;
     c.stream = static_cast<hipStream_t>(stream);
+    // the CU count (a device attribute, no allocation) sizes launch choices
+    if (hipDeviceGetAttribute(&c.cu_count, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return fail(nullptr, PBF_E_HIP, "pbf_run_noalloc: hipDeviceGetAttribute failed");
     size_t off = 0;
     auto take = [&](size_t bytes) {
         unsigned char* p = base + off;

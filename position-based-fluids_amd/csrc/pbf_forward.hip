@@ -71,20 +71,38 @@ __global__ __launch_bounds__(kPredictThreads) void k_predict_hash(
         s_bad[1] = 0u;
     }
     __syncthreads();
+    // every load of the thread's particles first, so that they are in flight
+    // together (computed item by item, each waited for its own loads)
+    float4 xa[kPredictItems], va[kPredictItems];
+    uint32_t ok[kPredictItems], okp[kPredictItems];
+#pragma unroll
+    for (int k = 0; k < kPredictItems; ++k) {
+        // clamped into the tile (be.y < be.z): no branch between the loads
+        const uint32_t i = min(be.y + (uint32_t)(k * kPredictThreads) + threadIdx.x, be.z - 1u);
+        xa[k] = x[i];
+        va[k] = v[i];
+    }
+    if (reset_old) {
+#pragma unroll
+        for (int k = 0; k < kPredictItems; ++k) {
+            const uint32_t i = min(be.y + (uint32_t)(k * kPredictThreads) + threadIdx.x, be.z - 1u);
+            ok[k] = old_keys[i];
+            okp[k] = old_keys[i > 0 ? i - 1 : 0];
+        }
+    }
 #pragma unroll
     for (int k = 0; k < kPredictItems; ++k) {
         const uint32_t i = be.y + (uint32_t)(k * kPredictThreads) + threadIdx.x;
         if (i < be.z) {
-            const float4 p = predicted_from(x[i], v[i], c, pause);
+            const float4 p = predicted_from(xa[k], va[k], c, pause);
             const uint32_t key = key_of(p, c);
             keys[i] = key;
             for (uint32_t q = 0; q < passes; ++q)
                 atomicAdd(&s_hist[q * kRadix + ((key >> (kRadixBits * q)) & (kRadix - 1))], 1u);
-            if (reset_old) {
-                // reset_grid.cl: empty the cells of the previous step
-                const uint32_t ok = old_keys[i];
-                if ((i == 0 || old_keys[i - 1] != ok) && ok < total_cells) cells[ok] = make_uint2(kEmpty, kEmpty);
-            }
+            // reset_grid.cl: empty the cells of the previous step (the first
+            // particle of each old cell; i = 0 always starts one)
+            if (reset_old && (i == 0 || okp[k] != ok[k]) && ok[k] < total_cells)
+                cells[ok[k]] = make_uint2(kEmpty, kEmpty);
             if (!(fabsf(p.x) <= 3.4e38f && fabsf(p.y) <= 3.4e38f && fabsf(p.z) <= 3.4e38f)) {
                 atomicMin(&s_bad[0], i);
                 atomicAdd(&s_bad[1], 1u);

@@ -48,16 +48,16 @@ STAGE_BYTES = {
 # kernel sources whose hash is stamped here (KERNEL_SOURCES): with any other
 # source the line reports traffic as null.
 PMC_TRAFFIC = {
-    "source_sha16": "c962af6ae1491d69",
+    "source_sha16": "41fe31376a9aff2a",
     "bytes_per_launch": {
-        "applyViscosity": 223774563,
-        "applyVorticity": 229816143,
-        "buildFriendsList": 923022121,
-        "computeDelta": 205309267,
-        "computeScaling": 202276427,
-        "onesweep": 17288309,
-        "predictPositions": 43909799,
-        "sortParticles": 96384372,
+        "applyViscosity": 213931531,
+        "applyVorticity": 229979248,
+        "buildFriendsList": 921125149,
+        "computeDelta": 203913465,
+        "computeScaling": 201910919,
+        "onesweep": 17935667,
+        "predictPositions": 43901059,
+        "sortParticles": 96384061,
     },
 }
 

@@ -210,6 +210,17 @@ int pbf_set_workgroup_sizes(pbf_ctx *ctx, const uint32_t *sizes /* PBF_FAMILIES 
 /* Friends build variant: 1 stages each tile's 27-cell neighbourhood in LDS,
  * 0 reads candidates from global memory (L2).  Both give the same lists. */
 int pbf_set_friends_mode(pbf_ctx *ctx, int staged);
+/* Lanes per particle of the global-read friends build: 1 (one thread scans
+ * a particle's 27 cells), 2, 4, 8 or 16 (each lane scans its share of the 27
+ * cells; the lanes' lists are joined in the reference's scan order, so the
+ * lists are the same), or 0 (the default): 16 lanes when the scene has at
+ * most two 64-particle waves per CU (32,768 particles on 256 CUs), else 1.
+ * More lanes shorten the build of scenes too small to fill the device; the
+ * lanes build scans twice, so it is slower on larger scenes.  No
+ * counterpart in the reference
+ * (its buildFriendsList runs one work-item per particle,
+ * assets/kernels/build_friends_list.cl:28).  PBF_E_INVALID for other values. */
+int pbf_set_friends_lanes(pbf_ctx *ctx, uint32_t lanes);
 /* Tiles per workgroup (the particles each thread handles in turn) of every
  * family: 1 launches one workgroup per tile of the block table; k > 1
  * launches about tiles / k workgroups; 0 launches a persistent grid, as many
@@ -220,8 +231,10 @@ int pbf_set_friends_mode(pbf_ctx *ctx, int staged);
 int pbf_set_tiles_per_workgroup(pbf_ctx *ctx, const uint32_t *tiles /* PBF_FAMILIES */);
 /* Autotuner: times every candidate size for each family (and both friends
  * build variants), then 4 tiles per workgroup and the persistent grid at the
- * chosen sizes, over `steps` substeps from the current state (which is
- * restored afterwards).  A choice other than the default (256, global-read
+ * chosen sizes, and on scenes of at most four 64-particle waves per CU the
+ * friends build with 1, 4 and 16 lanes per particle (unless lanes were set
+ * with pbf_set_friends_lanes), over `steps` substeps from the current state
+ * (which is restored afterwards).  A choice other than the default (256, global-read
  * friends build, one tile per workgroup) is kept only when it is more than
  * 3% faster.  Returns the chosen sizes in `chosen` (may be NULL; pbf_stats
  * reports the variant and the tiles per workgroup). */
@@ -234,7 +247,8 @@ typedef struct pbf_stats_t {
     uint32_t workgroup_size[PBF_FAMILIES];
     uint32_t friends_staged;      /* friends build variant (pbf_set_friends_mode) */
     uint32_t tiles_per_workgroup[PBF_FAMILIES]; /* pbf_set_tiles_per_workgroup */
-    uint32_t reserved[2];
+    uint32_t friends_lanes;       /* lanes per particle the friends build runs with */
+    uint32_t reserved[1];
 } pbf_stats_t;
 int pbf_stats(pbf_ctx *ctx, pbf_stats_t *out);
 

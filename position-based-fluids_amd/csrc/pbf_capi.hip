@@ -827,6 +827,7 @@ int pbf_stats(pbf_ctx* ctx, pbf_stats_t* out) {
     memset(out, 0, sizeof *out);
     out->unstaged_tiles = w[4];
     out->friends_staged = ctx->friends_staged ? 1u : 0u;
+    out->friends_lanes = ctx->friends_staged ? 1u : pbf::friends_lanes_for(ctx);
     out->sort_passes = ctx->sort.passes;
     out->sort_tiles = ctx->sort.tiles;
     out->total_cells = ctx->total_cells;
@@ -844,6 +845,16 @@ int pbf_set_friends_mode(pbf_ctx* ctx, int staged) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     drop_graphs(ctx);
     ctx->friends_staged = staged != 0;
+    return PBF_OK;
+}
+
+int pbf_set_friends_lanes(pbf_ctx* ctx, uint32_t lanes) {
+    if (!ctx) return PBF_E_INVALID;
+    if (!(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16))
+        return fail(ctx, PBF_E_INVALID, "friends lanes %u is not one of 0, 1, 2, 4, 8, 16", lanes);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    ctx->friends_lanes = lanes;
     return PBF_OK;
 }
 
@@ -991,6 +1002,30 @@ int pbf_autotune(pbf_ctx* ctx, uint32_t steps, uint32_t* chosen) {
     for (int f = 0; f < pbf::F_COUNT; ++f) {
         ctx->tiles_per_wg[f] = best_t[f];
         if (chosen) chosen[f] = 1u << best_l[f];
+    }
+    // lanes per particle of the global-read friends build, on scenes of at
+    // most four 64-particle waves per CU (above, one lane is faster): 1, 4
+    // and 16 against the default choice
+    if (rc == PBF_OK && !ctx->friends_staged && ctx->friends_lanes == 0 &&
+        ((uint64_t)ctx->n + 63) / 64 <= 4ull * (uint64_t)std::max(ctx->cu_count, 1)) {
+        const uint32_t dflt = pbf::friends_lanes_for(ctx);
+        double ms[pbf::F_COUNT];
+        if ((rc = trial_ms(ms)) == PBF_OK) {
+            const double at_default = ms[pbf::F_FRIENDS];
+            double best_ms = at_default;
+            uint32_t best_lanes = 0;
+            static const uint32_t kLaneTrials[3] = {1, 4, 16};
+            for (int k = 0; k < 3 && rc == PBF_OK; ++k) {
+                if (kLaneTrials[k] == dflt) continue;
+                ctx->friends_lanes = kLaneTrials[k];
+                if ((rc = trial_ms(ms)) != PBF_OK) break;
+                if (ms[pbf::F_FRIENDS] < kMargin * at_default && ms[pbf::F_FRIENDS] < best_ms) {
+                    best_ms = ms[pbf::F_FRIENDS];
+                    best_lanes = kLaneTrials[k];
+                }
+            }
+            ctx->friends_lanes = best_lanes;
+        }
     }
     if (rc == PBF_OK) rc = restore();
     (void)hipStreamSynchronize(ctx->stream);

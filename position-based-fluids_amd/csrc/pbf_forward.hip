@@ -497,6 +497,141 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
     }
 }
 
+// The friends build with T lanes per particle, for scenes too small to fill
+// the device with one thread per particle (the build's latency is then that
+// of one thread scanning its 27 cells).  Each workgroup takes a 64-particle
+// tile with 64 T threads; lane l of a particle scans its offsets
+// [27 l / T, 27 (l + 1) / T) in the x, y, z order, so the particle's list is
+// its lanes' lists one after another.  The lanes first count their friends
+// per circle, then take exclusive prefix sums of the counts across the
+// particle's lanes, then scan again and store each friend at its place in
+// the circle (friends past the 50 cap are dropped, as in the one-lane
+// build).  The first lane of the particle then stores the counts, regroups
+// the processed circles (pack_friends) and runs the first computeScaling.
+// Five per-circle counters in registers, read and advanced at a circle
+// chosen per lane (selects, not an indexed array).
+struct CircleCounts {
+    uint32_t v[kCircles] = {0u, 0u, 0u, 0u, 0u};
+    __device__ __forceinline__ uint32_t at(int q) const {
+        uint32_t r = v[0];
+#pragma unroll
+        for (int k = 1; k < kCircles; ++k) r = q == k ? v[k] : r;
+        return r;
+    }
+    __device__ __forceinline__ void inc(int q) {
+#pragma unroll
+        for (int k = 0; k < kCircles; ++k) v[k] += q == k ? 1u : 0u;
+    }
+};
+
+// Lane offsets [k0, k1) of particle i in the x, y, z order, candidates four
+// at a time; each friend of circle q advances cnt[q] and, with STORE, is
+// stored at place base[q] + its index among the lane's circle-q friends (if
+// under the 50 cap).
+template <bool STORE>
+__device__ __forceinline__ void scan_offsets(const float4& p, uint32_t i, uint32_t scan, const NeighbourHash& nh,
+                                             int k0, int k1, const uint2* __restrict__ cells,
+                                             const float4* __restrict__ xs, uint32_t n, const SceneConsts& c,
+                                             CircleCounts& cnt, const CircleCounts base, uint32_t* __restrict__ list,
+                                             uint32_t* __restrict__ sweep) {
+    for (int k = k0; k < k1; ++k) {
+        if (!(scan & (1u << k))) continue;
+        // the neighbour's hash stepped from the centre's dilated coordinates
+        // (NeighbourHash's steps), chosen per lane: k differs between a
+        // particle's lanes, and an indexed choice would go through memory
+        auto step = [](uint32_t e, int a) {
+            constexpr uint32_t kM = NeighbourHash::kM;
+            return a == 0 ? ((e - 1u) & kM) : (a == 1 ? e : (((e | ~kM) + 1u) & kM));
+        };
+        const uint32_t m = step(nh.x[1], k / 9) | (step(nh.y[1], (k / 3) % 3) << 1) | (step(nh.z[1], k % 3) << 2);
+        const uint2 b = cells[c.cell_base + (c.gbs_mask ? (m & c.gbs_mask) : (m % c.gbs))];
+        if (b.x == kEmpty) continue;
+        const uint32_t e = min(b.y, n - 1);
+        for (uint32_t j = b.x; j <= e; j += 4) {
+            const uint32_t r = min(e - j, 3u);
+            const float4* __restrict__ g = xs + j;
+            const float4 q[4] = {g[0], g[min(1u, r)], g[min(2u, r)], g[min(3u, r)]};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int circ = friend_circle(p, q[t], c);
+                if ((uint32_t)t > r || circ < 0 || j + t == i) continue;
+                if (STORE) {
+                    const uint32_t at = base.at(circ) + cnt.at(circ);
+                    if (at < (uint32_t)kPerCircle) {
+                        if (circ == 0)
+                            sweep[sweep_quad(i, (int)(at >> 2)) * 4 + (at & 3u)] = j + t;
+                        else
+                            list[friend_slot(i, circ, (int)at)] = j + t;
+                    }
+                }
+                cnt.inc(circ);
+            }
+        }
+    }
+}
+
+template <int T, bool LAMBDA0>
+__global__ __launch_bounds__(64 * T) void k_friends_lanes(const SceneConsts* __restrict__ scenes,
+                                                         const uint4* __restrict__ blocks, float4* __restrict__ xs,
+                                                         const uint2* __restrict__ cells,
+                                                         uint32_t* __restrict__ nb_cnt, uint32_t* __restrict__ nb_list,
+                                                         uint4* __restrict__ quads, float* __restrict__ rho,
+                                                         float* __restrict__ lambda, uint32_t n, uint32_t fresh,
+                                                         uint32_t* err) {
+    static_assert(T >= 2 && T <= 16 && (T & (T - 1)) == 0, "T lanes per particle: a power of two up to 16");
+    const uint4 be = blocks[xcd_tile(blockIdx.x, gridDim.x)];   // a tile of at most 64 particles
+    const SceneConsts c = scenes[be.x];
+    const int l = (int)(threadIdx.x % T);
+    const uint32_t i = be.y + threadIdx.x / T;
+    // a particle's T lanes are consecutive in one wave; lanes of particles
+    // past the tile take part in the shuffles with nothing to scan
+    const bool valid = i < be.z;
+    const uint32_t ii = valid ? i : be.y;
+    const float4 p = xs[ii];
+    const int32_t cx = cell_coord(p.x / c.h), cy = cell_coord(p.y / c.h), cz = cell_coord(p.z / c.h);
+    const uint32_t scan = valid ? cells_to_scan(p, cx, cy, cz, c, fresh) : 0u;
+    const NeighbourHash nh(cx, cy, cz);
+    const int k0 = 27 * l / T, k1 = 27 * (l + 1) / T;
+    CircleCounts cnt;
+    scan_offsets<false>(p, ii, scan, nh, k0, k1, cells, xs, n, c, cnt, cnt, nullptr, nullptr);
+    // friends of the particle's earlier lanes, per circle, and its totals
+    CircleCounts base, total;
+#pragma unroll
+    for (int q = 0; q < kCircles; ++q) {
+        uint32_t incl = cnt.v[q];
+#pragma unroll
+        for (int o = 1; o < T; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, T);
+            if (l >= o) incl += u;
+        }
+        base.v[q] = incl - cnt.v[q];
+        total.v[q] = __shfl(incl, T - 1, T);
+        cnt.v[q] = 0u;
+    }
+    uint32_t* list = nb_list;
+    uint32_t* sweep = reinterpret_cast<uint32_t*>(quads);
+    scan_offsets<true>(p, ii, scan, nh, k0, k1, cells, xs, n, c, cnt, base, list, sweep);
+    // the lanes' stores are complete before the first lane reads the
+    // circles back (the lanes share a wave and its L1)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    if (!valid || l != 0) return;
+    uint32_t packed = 0, dropped = 0;
+#pragma unroll
+    for (int q = 0; q < kCircles; ++q) {
+        const uint32_t kept = min(total.v[q], (uint32_t)kPerCircle);
+        packed |= kept << (6 * q);
+        dropped += total.v[q] - kept;
+    }
+    nb_cnt[i] = packed;
+    if (dropped) {
+        atomicOr(err, (uint32_t)PBF_FLAG_FRIENDS_OVERFLOW);
+        atomicAdd((unsigned long long*)(err + 2), (unsigned long long)dropped);
+        log_event(err, PBF_LOG_FRIENDS_OVERFLOW, i, dropped, packed);
+    }
+    pack_friends(i, packed, reinterpret_cast<const uint4*>(list), quads);
+    if (LAMBDA0) scaling_of<false>(i, packed, xs, quads, rho, lambda, nullptr, c, [&](uint32_t j) { return xs[j]; });
+}
+
 // Staging budget of the LDS variant per workgroup size: box cells, particles.
 template <int BS> struct FriendsBudget;
 template <> struct FriendsBudget<64> { static constexpr int cells = 128, staged = 1024; };
@@ -979,13 +1114,46 @@ hipError_t launch_gather(pbf_ctx* ctx) {
         });                                                                                                    \
     } while (0)
 
+uint32_t friends_lanes_for(const pbf_ctx* ctx) {
+    if (ctx->friends_lanes) return ctx->friends_lanes;
+    // The lanes build scans twice and stores friends one by one, so it pays
+    // only while one lane per particle leaves most of the device idle: 16
+    // lanes up to 2 waves per CU (32,768 particles on 256 CUs; measured at
+    // 4,096 and 16,384 particles), one lane above (slower with any count of
+    // lanes from 65,536 particles on).
+    const uint64_t waves = ((uint64_t)ctx->n + 63) / 64;
+    return waves <= 2ull * (uint64_t)std::max(ctx->cu_count, 1) ? 16u : 1u;
+}
+
 hipError_t launch_friends(pbf_ctx* ctx, bool lambda0) {
     track_begin(ctx, "buildFriendsList");
     const uint32_t fresh = ctx->cells_fresh ? 1u : 0u;
     const BlockTable& t = table_for(ctx, F_FRIENDS);
     float4* xs = ctx->xs[ctx->xs_cur];
     const uint2* cells = reinterpret_cast<const uint2*>(ctx->cells);
-    if (ctx->friends_staged) {
+    const uint32_t lanes = ctx->friends_staged ? 1u : friends_lanes_for(ctx);
+    if (lanes > 1) {
+        // 64-particle tiles, 64 * lanes threads each
+        const BlockTable& t64 = ctx->bt[0];
+#define PBF_LAUNCH_LANES(T)                                                                                        \
+    do {                                                                                                           \
+        if (lambda0)                                                                                               \
+            PBF_KERNEL((k_friends_lanes<T, true>), dim3(t64.count), dim3(64 * T), 0, ctx->stream, ctx->sc_d, t64.d, \
+                       xs, cells, ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->rho, ctx->lambda, ctx->n, fresh,    \
+                       ctx->err);                                                                                  \
+        else                                                                                                       \
+            PBF_KERNEL((k_friends_lanes<T, false>), dim3(t64.count), dim3(64 * T), 0, ctx->stream, ctx->sc_d,      \
+                       t64.d, xs, cells, ctx->nb_cnt, ctx->nb_list, ctx->nb_quads, ctx->rho, ctx->lambda, ctx->n,  \
+                       fresh, ctx->err);                                                                           \
+    } while (0)
+        switch (lanes) {
+            case 2: PBF_LAUNCH_LANES(2); break;
+            case 4: PBF_LAUNCH_LANES(4); break;
+            case 8: PBF_LAUNCH_LANES(8); break;
+            default: PBF_LAUNCH_LANES(16); break;
+        }
+#undef PBF_LAUNCH_LANES
+    } else if (ctx->friends_staged) {
         PBF_DISPATCH_BS(ctx->bs_log2[F_FRIENDS], {
             if (lambda0)
                 PBF_KERNEL((k_friends<BS, true>), dim3(t.count), dim3(BS), 0, ctx->stream, ctx->sc_d, t.d, xs,

@@ -107,6 +107,9 @@ struct pbf_ctx {
     pbf::BlockTable bt_predict;       // 256*kPredictItems particles per block
     int bs_log2[pbf::F_COUNT] = {8, 8, 8, 8, 8};  // chosen workgroup sizes
     bool friends_staged = false;      // friends build: LDS-staged candidates or global reads
+    // lanes per particle of the global-read friends build: 1, 2, 4, 8 or 16,
+    // or 0 to choose from the particle count (friends_lanes_for)
+    uint32_t friends_lanes = 0;
     // tiles per workgroup of each family: 1 = one workgroup per tile, k > 1 =
     // workgroups striding over k tiles each, 0 = a persistent grid of
     // resident[f][size] workgroups (the device's capacity for that kernel)
@@ -177,6 +180,9 @@ hipError_t launch_sort(pbf_ctx* ctx);
 hipError_t launch_gather(pbf_ctx* ctx);
 // lambda0: the friends build also runs the substep's first computeScaling
 hipError_t launch_friends(pbf_ctx* ctx, bool lambda0 = false);
+// Lanes per particle the friends build runs with (ctx->friends_lanes, or
+// when 0 a choice from the particle count and the CU count).
+uint32_t friends_lanes_for(const pbf_ctx* ctx);
 // fills ctx->resident from the kernels' occupancy on ctx->cu_count CUs
 hipError_t query_residency(pbf_ctx* ctx);
 // aux / fut: per-particle records for the adjoint (nullptr: plain step)

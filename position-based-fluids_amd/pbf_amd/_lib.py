@@ -138,7 +138,8 @@ class PbfStats(ctypes.Structure):
         ("workgroup_size", ctypes.c_uint32 * 5),
         ("friends_staged", ctypes.c_uint32),
         ("tiles_per_workgroup", ctypes.c_uint32 * 5),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("friends_lanes", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 1),
     ]
 
 
@@ -177,6 +178,7 @@ _SIGNATURES = {
     "pbf_reset_timings": (ctypes.c_int, [_VP]),
     "pbf_set_workgroup_sizes": (ctypes.c_int, [_VP, _UP]),
     "pbf_set_friends_mode": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "pbf_set_friends_lanes": (ctypes.c_int, [_VP, ctypes.c_uint32]),
     "pbf_set_tiles_per_workgroup": (ctypes.c_int, [_VP, _UP]),
     "pbf_autotune": (ctypes.c_int, [_VP, ctypes.c_uint32, _UP]),
     "pbf_stats": (ctypes.c_int, [_VP, ctypes.POINTER(PbfStats)]),

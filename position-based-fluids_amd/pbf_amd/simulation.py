@@ -189,6 +189,13 @@ class Simulation:
         m = self.FRIENDS_MODES[mode] if isinstance(mode, str) else (1 if mode else 0)
         check(lib().pbf_set_friends_mode(self._ctx, m), self._ctx)
 
+    def set_friends_lanes(self, lanes: int):
+        """Lanes per particle of the global-read friends build
+        (pbf_set_friends_lanes): 1, 2, 4, 8, 16, or 0 to choose from the
+        particle count.  The lists do not depend on it."""
+        self._ensure()
+        check(lib().pbf_set_friends_lanes(self._ctx, int(lanes)), self._ctx)
+
     def set_tiles_per_workgroup(self, tiles: dict):
         """Tiles each workgroup of a family handles in turn: 1 (one per tile),
         k > 1, or 0 for a persistent grid of resident workgroups."""

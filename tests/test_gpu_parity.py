@@ -643,6 +643,63 @@ def test_per_particle_friends_builds_match(mode, size):
     sim.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
+def test_friends_lanes_match(lanes):
+    """The global-read friends build with 1 to 16 lanes per particle gives
+    the oracle's lists: on the 16,384-particle dam break, on hash-folded
+    buckets (gridBufSize 97, visited once per alias) and on a cluster that
+    overflows the 50-per-circle cap (the first 50 in scan order are kept
+    across the lanes' shares, every further friend is counted as dropped)."""
+    scene = pbf_amd.config("dam16k", seed=21)
+    sim, ref = make_pair(scene)
+    sim.set_friends_lanes(lanes)
+    assert sim.stats()["friends_lanes"] == lanes
+    advance(sim, ref, 3)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()
+
+    scene = pbf_amd.dam_break(12, 12, 12, 0.1, (1.2, 1.2, 1.2), seed=11, grid_buf_size=97)
+    sim, ref = make_pair(scene)
+    sim.set_friends_lanes(lanes)
+    advance(sim, ref, 2)
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()
+
+    rng = np.random.default_rng(0)
+    n = 400
+    x = np.zeros((n, 4), np.float32)
+    x[:, :3] = (0.55 + 0.02 * rng.random((n, 3))).astype(np.float32)
+    p = pbf_amd.physical_params(n, 0.1, (0, 0, 0), (1.0, 1.0, 1.0), iters=1)
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(), x, np.zeros_like(x))
+    sim, ref = make_pair(scene)
+    sim.set_friends_lanes(lanes)
+    advance(sim, ref, 1)
+    st = check_structure(sim, ref)
+    flags, dropped = sim.error_flags(reset=True)
+    counts = st["friends"][: 5 * n].reshape(5, n)
+    assert flags & pbf_amd.PBF_FLAG_FRIENDS_OVERFLOW
+    assert dropped == int(n * (n - 1) - counts.sum())
+    sim.close()
+
+
+def test_friends_lanes_default_and_validation():
+    """Without a setting the build takes 16 lanes per particle up to two
+    64-particle waves per CU (16 at 16,384 particles on the 256-CU device), 1
+    above (1M); other counts than 0, 1, 2, 4, 8, 16 are refused."""
+    for name, want in (("dam16k", 16), ("dam1m", 1)):
+        scene = pbf_amd.config(name, seed=0)
+        sim = pbf_amd.Simulation(scene.params, scene.options)
+        sim.set_state(scene.x, scene.v)
+        assert sim.stats()["friends_lanes"] == want, name
+        sim.set_friends_mode("staged")   # the LDS-staged build runs one lane per particle
+        assert sim.stats()["friends_lanes"] == 1
+        with pytest.raises(pbf_amd.PbfError):
+            sim.set_friends_lanes(3)
+        sim.close()
+
+
 def test_fast_sqrt_and_reciprocal_are_ieee():
     # every normal binary32 input: the kernels' square root and reciprocal
     # equal sqrtf and 1.0f / x

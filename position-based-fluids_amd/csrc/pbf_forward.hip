@@ -524,30 +524,45 @@ struct CircleCounts {
     }
 };
 
-// Lane offsets [k0, k1) of particle i in the x, y, z order, candidates four
-// at a time; each friend of circle q advances cnt[q] and, with STORE, is
-// stored at place base[q] + its index among the lane's circle-q friends (if
-// under the 50 cap).
-template <bool STORE>
-__device__ __forceinline__ void scan_offsets(const float4& p, uint32_t i, uint32_t scan, const NeighbourHash& nh,
-                                             int k0, int k1, const uint2* __restrict__ cells,
-                                             const float4* __restrict__ xs, uint32_t n, const SceneConsts& c,
-                                             CircleCounts& cnt, const CircleCounts base, uint32_t* __restrict__ list,
-                                             uint32_t* __restrict__ sweep) {
-    for (int k = k0; k < k1; ++k) {
-        if (!(scan & (1u << k))) continue;
-        // the neighbour's hash stepped from the centre's dilated coordinates
-        // (NeighbourHash's steps), chosen per lane: k differs between a
-        // particle's lanes, and an indexed choice would go through memory
-        auto step = [](uint32_t e, int a) {
-            constexpr uint32_t kM = NeighbourHash::kM;
-            return a == 0 ? ((e - 1u) & kM) : (a == 1 ? e : (((e | ~kM) + 1u) & kM));
-        };
-        const uint32_t m = step(nh.x[1], k / 9) | (step(nh.y[1], (k / 3) % 3) << 1) | (step(nh.z[1], k % 3) << 2);
-        const uint2 b = cells[c.cell_base + (c.gbs_mask ? (m & c.gbs_mask) : (m % c.gbs))];
-        if (b.x == kEmpty) continue;
-        const uint32_t e = min(b.y, n - 1);
-        for (uint32_t j = b.x; j <= e; j += 4) {
+// Bucket ranges {first, last} of a lane's offsets [k0, k0 + U) (U at most
+// MAXOFF; a culled or missing one is empty), loaded together: the neighbour's
+// hash stepped from the centre's dilated coordinates (NeighbourHash's steps)
+// chosen per lane (k differs between a particle's lanes; an indexed choice
+// would go through memory), each range one 8-byte load.
+template <int MAXOFF>
+__device__ __forceinline__ void load_ranges(uint32_t scan, const NeighbourHash& nh, int k0, int k1,
+                                            const uint2* __restrict__ cells, const SceneConsts& c,
+                                            uint64_t* rng) {
+    auto step = [](uint32_t e, int a) {
+        constexpr uint32_t kM = NeighbourHash::kM;
+        return a == 0 ? ((e - 1u) & kM) : (a == 1 ? e : (((e | ~kM) + 1u) & kM));
+    };
+#pragma unroll
+    for (int u = 0; u < MAXOFF; ++u) {
+        const int k = k0 + u;
+        rng[u] = 0xFFFFFFFFFFFFFFFFull;
+        if (k < k1 && (scan & (1u << k))) {
+            const uint32_t m = step(nh.x[1], k / 9) | (step(nh.y[1], (k / 3) % 3) << 1) | (step(nh.z[1], k % 3) << 2);
+            rng[u] = reinterpret_cast<const uint64_t*>(cells)[c.cell_base + (c.gbs_mask ? (m & c.gbs_mask)
+                                                                                          : (m % c.gbs))];
+        }
+    }
+}
+
+// The buckets of a lane's ranges in order, candidates four at a time; each
+// friend of circle q advances cnt[q] and, with STORE, is stored at place
+// base[q] + its index among the lane's circle-q friends (if under the 50
+// cap).
+template <int MAXOFF, bool STORE>
+__device__ __forceinline__ void scan_ranges(const float4& p, uint32_t i, const uint64_t* rng,
+                                            const float4* __restrict__ xs, uint32_t n, const SceneConsts& c,
+                                            CircleCounts& cnt, const CircleCounts base, uint32_t* __restrict__ list,
+                                            uint32_t* __restrict__ sweep) {
+#pragma unroll
+    for (int u = 0; u < MAXOFF; ++u) {
+        const uint32_t bs = (uint32_t)rng[u], e = min((uint32_t)(rng[u] >> 32), n - 1);
+        if (bs == kEmpty) continue;
+        for (uint32_t j = bs; j <= e; j += 4) {
             const uint32_t r = min(e - j, 3u);
             const float4* __restrict__ g = xs + j;
             const float4 q[4] = {g[0], g[min(1u, r)], g[min(2u, r)], g[min(3u, r)]};
@@ -592,8 +607,11 @@ __global__ __launch_bounds__(64 * T) void k_friends_lanes(const SceneConsts* __r
     const uint32_t scan = valid ? cells_to_scan(p, cx, cy, cz, c, fresh) : 0u;
     const NeighbourHash nh(cx, cy, cz);
     const int k0 = 27 * l / T, k1 = 27 * (l + 1) / T;
+    constexpr int kMaxOff = (27 + T - 1) / T;
+    uint64_t rng[kMaxOff];
+    load_ranges<kMaxOff>(scan, nh, k0, k1, cells, c, rng);
     CircleCounts cnt;
-    scan_offsets<false>(p, ii, scan, nh, k0, k1, cells, xs, n, c, cnt, cnt, nullptr, nullptr);
+    scan_ranges<kMaxOff, false>(p, ii, rng, xs, n, c, cnt, cnt, nullptr, nullptr);
     // friends of the particle's earlier lanes, per circle, and its totals
     CircleCounts base, total;
 #pragma unroll
@@ -610,7 +628,7 @@ __global__ __launch_bounds__(64 * T) void k_friends_lanes(const SceneConsts* __r
     }
     uint32_t* list = nb_list;
     uint32_t* sweep = reinterpret_cast<uint32_t*>(quads);
-    scan_offsets<true>(p, ii, scan, nh, k0, k1, cells, xs, n, c, cnt, base, list, sweep);
+    scan_ranges<kMaxOff, true>(p, ii, rng, xs, n, c, cnt, base, list, sweep);
     // the lanes' stores are complete before the first lane reads the
     // circles back (the lanes share a wave and its L1)
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");

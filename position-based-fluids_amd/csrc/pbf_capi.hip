@@ -279,7 +279,7 @@ int create_impl(const pbf_params* params, const pbf_options* opts, uint32_t S, p
         ctx->bt[k].count = (uint32_t)t.size();
     }
     {
-        const std::vector<uint4> t = block_table(ctx, 256u * pbf::kPredictItems);
+        const std::vector<uint4> t = block_table(ctx, (uint32_t)pbf::kPredictTile);
         CREATE_TRY(dalloc(&ctx->bt_predict.d, t.size()));
         if (!t.empty())
             CREATE_TRY(hipMemcpy(ctx->bt_predict.d, t.data(), sizeof(uint4) * t.size(), hipMemcpyHostToDevice));

@@ -33,7 +33,6 @@ namespace pbf {
 
 namespace {
 
-constexpr int kPredictThreads = 256;
 
 __device__ __forceinline__ float4 predicted_from(const float4 x, const float4 v, const SceneConsts& c, uint32_t pause) {
     // predict_positions.cl:12-17

@@ -31,7 +31,9 @@ constexpr int kErrWords = 8;
 // end, 0} per workgroup, so a workgroup never straddles two scenes.  Tables
 // exist for these workgroup sizes (index = log2(size) - 6).
 constexpr int kNumBlockSizes = 5;  // 64, 128, 256, 512, 1024
-constexpr int kPredictItems = 8;   // particles per thread in k_predict_hash
+constexpr int kPredictThreads = 1024;  // threads of a k_predict_hash workgroup
+constexpr int kPredictItems = 4;      // particles per thread in k_predict_hash
+constexpr int kPredictTile = kPredictThreads * kPredictItems;   // particles per block
 
 struct BlockTable {
     uint4* d = nullptr;
@@ -104,7 +106,7 @@ struct pbf_ctx {
     hipStream_t stream = nullptr;
 
     pbf::BlockTable bt[pbf::kNumBlockSizes];
-    pbf::BlockTable bt_predict;       // 256*kPredictItems particles per block
+    pbf::BlockTable bt_predict;       // kPredictTile particles per block
     int bs_log2[pbf::F_COUNT] = {8, 8, 8, 8, 8};  // chosen workgroup sizes
     bool friends_staged = false;      // friends build: LDS-staged candidates or global reads
     // lanes per particle of the global-read friends build: 1, 2, 4, 8 or 16,

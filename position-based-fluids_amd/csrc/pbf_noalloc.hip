@@ -52,7 +52,7 @@ constexpr size_t scratch_bytes(uint32_t n, uint32_t cells) {
     b += align_up(4 * (size_t)kErrAllocWords);
     b += align_up(sizeof(SceneConsts));
     for (int k = 0; k < kNumBlockSizes; ++k) b += align_up(16 * (((size_t)n + (64u << k) - 1) / (64u << k)));
-    b += align_up(16 * (((size_t)n + 256 * kPredictItems - 1) / (256 * kPredictItems)));
+    b += align_up(16 * (((size_t)n + kPredictTile - 1) / kPredictTile));
     return b;
 }
 
@@ -182,7 +182,7 @@ extern "C" int pbf_run_noalloc(const pbf_params* params, const pbf_options* opt,
         c.bt[k].count = (n + per - 1) / per;
         c.bt[k].d = reinterpret_cast<uint4*>(take(16 * (size_t)c.bt[k].count));
     }
-    c.bt_predict.count = (n + 256 * kPredictItems - 1) / (256 * kPredictItems);
+    c.bt_predict.count = (n + kPredictTile - 1) / kPredictTile;
     c.bt_predict.d = reinterpret_cast<uint4*>(take(16 * (size_t)c.bt_predict.count));
     if (off > kScratchBytes)   // the layout above and scratch_bytes disagree
         return fail(nullptr, PBF_E_INVALID, "pbf_run_noalloc: layout of %zu bytes exceeds the scratch (%zu)", off,
@@ -194,7 +194,7 @@ extern "C" int pbf_run_noalloc(const pbf_params* params, const pbf_options* opt,
         hipLaunchKernelGGL(k_fill_blocks, dim3((c.bt[k].count + 255) / 256), dim3(256), 0, c.stream, c.bt[k].d,
                            c.bt[k].count, n, 64u << k);
     hipLaunchKernelGGL(k_fill_blocks, dim3((c.bt_predict.count + 255) / 256), dim3(256), 0, c.stream, c.bt_predict.d,
-                       c.bt_predict.count, n, 256u * kPredictItems);
+                       c.bt_predict.count, n, (uint32_t)kPredictTile);
     hipLaunchKernelGGL(k_fill_words, dim3(grid_for(2 * (size_t)c.total_cells)), dim3(256), 0, c.stream, c.cells,
                        2 * (size_t)c.total_cells, 0xFFFFFFFFu);
     hipLaunchKernelGGL(k_fill_words, dim3(grid_for(n)), dim3(256), 0, c.stream, c.nb_cnt, (size_t)n, 0u);

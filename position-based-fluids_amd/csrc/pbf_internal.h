@@ -13,9 +13,9 @@
 
 namespace pbf {
 
-// Onesweep radix sort geometry: 256 threads, 16 keys per thread, 8-bit digits.
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+// Onesweep radix sort geometry: 1024 threads, 4 keys per thread, 8-bit digits.
+constexpr int kSortThreads = 1024;
+constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;

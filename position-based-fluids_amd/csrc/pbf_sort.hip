@@ -42,9 +42,11 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint32_t* __res
                                                            uint32_t shift, const uint32_t* __restrict__ hist,
                                                            uint32_t* status, uint32_t* tile_ctr,
                                                            uint32_t* err) {
-    __shared__ uint32_t s_cnt[kSortThreads / 64][kRadix];
+    constexpr int kWaves = kSortThreads / 64;
+    static_assert(kSortThreads >= kRadix && kRadix % 64 == 0, "one thread per digit in the first waves");
+    __shared__ uint32_t s_cnt[kWaves][kRadix];
     __shared__ uint32_t s_off[kRadix];
-    __shared__ uint32_t s_wsum[kSortThreads / 64];
+    __shared__ uint32_t s_wsum[kRadix / 64];
     __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x;
@@ -52,7 +54,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint32_t* __res
     const int lane = tid & 63;
 
     if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
-    for (int q = tid; q < (kSortThreads / 64) * kRadix; q += kSortThreads) (&s_cnt[0][0])[q] = 0;
+    for (int q = tid; q < kWaves * kRadix; q += kSortThreads) (&s_cnt[0][0])[q] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
 
@@ -90,50 +92,56 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint32_t* __res
     }
     __syncthreads();
 
-    // One thread per digit: wave-exclusive offsets, tile count, look-back.
+    // The first kRadix threads, one per digit: wave-exclusive offsets, the
+    // tile's count, the global base and the look-back.
     const uint32_t d = (uint32_t)tid;
-    const uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
-    const uint32_t tot = c0 + c1 + c2 + c3;
-    s_cnt[0][d] = 0;
-    s_cnt[1][d] = c0;
-    s_cnt[2][d] = c0 + c1;
-    s_cnt[3][d] = c0 + c1 + c2;
-
-    uint32_t* my = status + (size_t)tile * kRadix + d;
-    __hip_atomic_store(my, (tile == 0 ? kFlagPre : kFlagAgg) | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-
-    // Global base of digit d: exclusive scan of this pass's histogram.
-    const uint32_t hv = hist[d];
-    const uint32_t incl = wave_inclusive_scan(hv, lane);
-    if (lane == 63) s_wsum[w] = incl;
-    __syncthreads();
-    uint32_t gbase = incl - hv;
-    for (int q = 0; q < w; ++q) gbase += s_wsum[q];
-
-    uint32_t excl = 0;
-    if (tile > 0) {
-        int t = (int)tile - 1;
-        uint32_t spins = 0;
-        while (t >= 0) {
-            const uint32_t s = __hip_atomic_load(status + (size_t)t * kRadix + d, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t flag = s & ~kValMask;
-            if (flag == 0u) {
-                if (++spins > kSpinLimit) {
-                    atomicOr(err, (uint32_t)PBF_FLAG_SORT_TIMEOUT);
-                    log_event(err, PBF_LOG_SORT_TIMEOUT, tile, d, shift);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            excl += s & kValMask;
-            if (flag == kFlagPre) break;
-            --t;
+    const bool digit = tid < kRadix;
+    uint32_t tot = 0, hv = 0, incl = 0;
+    uint32_t* my = status + (size_t)tile * kRadix + (digit ? d : 0u);
+    if (digit) {
+#pragma unroll
+        for (int q = 0; q < kWaves; ++q) {
+            const uint32_t cq = s_cnt[q][d];
+            s_cnt[q][d] = tot;
+            tot += cq;
         }
-        __hip_atomic_store(my, kFlagPre | (excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(my, (tile == 0 ? kFlagPre : kFlagAgg) | tot, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        // global base of digit d: exclusive scan of this pass's histogram
+        hv = hist[d];
+        incl = wave_inclusive_scan(hv, lane);
+        if (lane == 63) s_wsum[w] = incl;
     }
-    s_off[d] = gbase + excl;
+    __syncthreads();
+    if (digit) {
+        uint32_t gbase = incl - hv;
+        for (int q = 0; q < w; ++q) gbase += s_wsum[q];
+
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int t = (int)tile - 1;
+            uint32_t spins = 0;
+            while (t >= 0) {
+                const uint32_t s = __hip_atomic_load(status + (size_t)t * kRadix + d, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t flag = s & ~kValMask;
+                if (flag == 0u) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(err, (uint32_t)PBF_FLAG_SORT_TIMEOUT);
+                        log_event(err, PBF_LOG_SORT_TIMEOUT, tile, d, shift);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += s & kValMask;
+                if (flag == kFlagPre) break;
+                --t;
+            }
+            __hip_atomic_store(my, kFlagPre | (excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_off[d] = gbase + excl;
+    }
     __syncthreads();
 
 #pragma unroll

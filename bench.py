@@ -48,16 +48,16 @@ STAGE_BYTES = {
 # kernel sources whose hash is stamped here (KERNEL_SOURCES): with any other
 # source the line reports traffic as null.
 PMC_TRAFFIC = {
-    "source_sha16": "41fe31376a9aff2a",
+    "source_sha16": "45f5e542146aa0d0",
     "bytes_per_launch": {
-        "applyViscosity": 213931531,
-        "applyVorticity": 229979248,
-        "buildFriendsList": 921125149,
-        "computeDelta": 203913465,
-        "computeScaling": 201910919,
-        "onesweep": 17935667,
-        "predictPositions": 43901059,
-        "sortParticles": 96384061,
+        "applyViscosity": 214665389,
+        "applyVorticity": 221356936,
+        "buildFriendsList": 920986122,
+        "computeDelta": 204380207,
+        "computeScaling": 202238567,
+        "onesweep": 17686760,
+        "predictPositions": 43444010,
+        "sortParticles": 96379833,
     },
 }
 

@@ -496,17 +496,6 @@ __global__ __launch_bounds__(BS) void k_friends_direct(const SceneConsts* __rest
     }
 }
 
-// The friends build with T lanes per particle, for scenes too small to fill
-// the device with one thread per particle (the build's latency is then that
-// of one thread scanning its 27 cells).  Each workgroup takes a 64-particle
-// tile with 64 T threads; lane l of a particle scans its offsets
-// [27 l / T, 27 (l + 1) / T) in the x, y, z order, so the particle's list is
-// its lanes' lists one after another.  The lanes first count their friends
-// per circle, then take exclusive prefix sums of the counts across the
-// particle's lanes, then scan again and store each friend at its place in
-// the circle (friends past the 50 cap are dropped, as in the one-lane
-// build).  The first lane of the particle then stores the counts, regroups
-// the processed circles (pack_friends) and runs the first computeScaling.
 // Five per-circle counters in registers, read and advanced at a circle
 // chosen per lane (selects, not an indexed array).
 struct CircleCounts {
@@ -584,6 +573,17 @@ __device__ __forceinline__ void scan_ranges(const float4& p, uint32_t i, const u
     }
 }
 
+// The friends build with T lanes per particle, for scenes too small to fill
+// the device with one thread per particle (the build's latency is then that
+// of one thread scanning its 27 cells).  Each workgroup takes a 64-particle
+// tile with 64 T threads; lane l of a particle scans its offsets
+// [27 l / T, 27 (l + 1) / T) in the x, y, z order, so the particle's list is
+// its lanes' lists one after another.  The lanes first count their friends
+// per circle, then take exclusive prefix sums of the counts across the
+// particle's lanes, then scan again and store each friend at its place in
+// the circle (friends past the 50 cap are dropped, as in the one-lane
+// build).  The first lane of the particle then stores the counts, regroups
+// the processed circles (pack_friends) and runs the first computeScaling.
 template <int T, bool LAMBDA0>
 __global__ __launch_bounds__(64 * T) void k_friends_lanes(const SceneConsts* __restrict__ scenes,
                                                          const uint4* __restrict__ blocks, float4* __restrict__ xs,

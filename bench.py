@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -45,30 +46,62 @@ STAGE_BYTES = {
 # three per substep plus the last, which also writes v and x);
 # profiles/pmc_traffic_dam1m.md holds the counts and commands (the
 # --pmc-summary mode of this script prints this table).  They belong to the
-# kernel sources whose hash is stamped here (KERNEL_SOURCES): with any other
-# source the line reports traffic as null.
+# kernel sources whose code hash is stamped here (KERNEL_SOURCES, comments
+# and indentation left out): with any other code the line reports traffic as
+# null.
 PMC_TRAFFIC = {
-    "source_sha16": "45f5e542146aa0d0",
+    "source_sha16": "db39748eb50d8e69",
     "bytes_per_launch": {
-        "applyViscosity": 214665389,
-        "applyVorticity": 221356936,
-        "buildFriendsList": 920986122,
-        "computeDelta": 204380207,
-        "computeScaling": 202238567,
-        "onesweep": 17686760,
-        "predictPositions": 43444010,
-        "sortParticles": 96379833,
+        "applyViscosity": 218489769,
+        "applyVorticity": 218930132,
+        "buildFriendsList": 926612343,
+        "computeDelta": 205198836,
+        "computeScaling": 202740603,
+        "onesweep": 17666780,
+        "predictPositions": 43443639,
+        "sortParticles": 96380407,
     },
 }
 
 KERNEL_SOURCES = ("pbf_forward.hip", "pbf_device.h", "pbf_sort.hip", "pbf_internal.h")
 
 
+def source_code_tokens(text: str) -> str:
+    """C/C++ source with its comments removed and each whitespace run made a
+    line break (if it holds one; preprocessor lines end there) or a space:
+    what the compiler sees, so an edit to comments or indentation alone
+    keeps the stamp.  String and character literals are kept whole."""
+    out, i, n = [], 0, len(text)
+    while i < n:
+        ch = text[i]
+        if ch in "\"'":
+            j = i + 1
+            while j < n and text[j] != ch:
+                j += 2 if text[j] == "\\" else 1
+            out.append(text[i:j + 1])
+            i = j + 1
+        elif text.startswith("//", i):
+            j = text.find("\n", i)
+            i = n if j < 0 else j
+            out.append(" ")
+        elif text.startswith("/*", i):
+            j = text.find("*/", i + 2)
+            i = n if j < 0 else j + 2
+            out.append(" ")
+        else:
+            out.append(ch)
+            i += 1
+    return re.sub(r"\s+", lambda m: "\n" if "\n" in m.group(0) else " ", "".join(out)).strip()
+
+
 def kernel_source_sha16() -> str:
+    """Hash of the kernel sources' code (comments and layout left out)."""
     import hashlib
     h = hashlib.sha256()
     for name in KERNEL_SOURCES:
-        h.update((ROOT / "position-based-fluids_amd" / "csrc" / name).read_bytes())
+        text = (ROOT / "position-based-fluids_amd" / "csrc" / name).read_text()
+        h.update(source_code_tokens(text).encode())
+        h.update(b"\0")
     return h.hexdigest()[:16]
 
 

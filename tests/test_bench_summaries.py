@@ -138,6 +138,17 @@ def test_traffic_table_of_other_sources_is_not_used(monkeypatch):
     assert bench.pmc_traffic("computeDelta", 4096) is None
 
 
+def test_source_stamp_ignores_comments_and_indentation_only():
+    a = '#define K 4\nint f(int x) {  // twice\n    return x * 2; /* a\n block */\n}\nconst char* s = "// kept";\n'
+    b = '#define K 4\n\n// a new comment line\nint f(int x) {\n  return x * 2;\n}\nconst char* s = "// kept";\n'
+    assert bench.source_code_tokens(a) == bench.source_code_tokens(b)
+    assert bench.source_code_tokens(a).splitlines()[0] == "#define K 4"
+    for changed in (a.replace("x * 2", "x * 3"), a.replace('"// kept"', '"// kept "'),
+                    a.replace("#define K 4\nint", "#define K 4 int")):
+        assert bench.source_code_tokens(changed) != bench.source_code_tokens(a)
+    assert bench.source_code_tokens("char c = '\\'';  // it's\nint y;") == "char c = '\\'';\nint y;"
+
+
 def test_a_run_without_markers_is_refused(tmp_path):
     d = tmp_path / "kt"
     d.mkdir()

@@ -127,7 +127,9 @@ int pbf_set_collision_quads(pbf_ctx *ctx, uint32_t scene, const pbf_quad *quads,
 int pbf_step(pbf_ctx *ctx, uint32_t pause);
 
 /* steps x Simulation::Step, as Runner::run does per frame
- * (src/Runner.cpp:124-132). */
+ * (src/Runner.cpp:124-132).  With graphs on (pbf_set_graph), groups of four
+ * substeps are replayed as one graph each, the rest one graph per substep;
+ * the results equal pbf_step called `steps` times. */
 int pbf_run(pbf_ctx *ctx, uint32_t steps, uint32_t pause);
 
 /* pbf_run for callers that cannot allocate device memory (code under HIP
@@ -148,8 +150,9 @@ int pbf_run(pbf_ctx *ctx, uint32_t steps, uint32_t pause);
 int pbf_run_noalloc(const pbf_params *params, const pbf_options *opt, const float *x4, const float *v4, uint32_t n,
                     uint32_t steps, float *x4_out, float *v4_out, void *stream);
 
-/* Replays each substep as an instantiated HIP graph (captured on first use
- * for every variant of the buffer rotation).  Ignored while profiling. */
+/* Replays each substep (pbf_step) or group of four substeps (pbf_run) as an
+ * instantiated HIP graph, captured on first use for every variant of the
+ * buffer rotation.  Ignored while profiling. */
 int pbf_set_graph(pbf_ctx *ctx, int enable);
 
 /* Blocks until all queued work of the context is done. */

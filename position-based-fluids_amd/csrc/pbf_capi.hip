@@ -366,22 +366,26 @@ uint32_t graph_key(const pbf_ctx* ctx, uint32_t pause) {
            (ctx->cells_hold_keys ? 8u : 0u) | (pause ? 16u : 0u) | ((uint32_t)ctx->xs_cur << 5);
 }
 
-int step_graph(pbf_ctx* ctx, uint32_t pause) {
+// Replays `steps` consecutive substeps as one graph, captured on first use
+// for each variant (the key of the first substep; the later ones follow from
+// it).
+int step_graph(pbf_ctx* ctx, uint32_t pause, uint32_t steps) {
     const uint32_t key = graph_key(ctx, pause);
     for (pbf::GraphEntry& g : ctx->graphs) {
-        if (g.key != key) continue;
+        if (g.key != key || g.steps != steps) continue;
         HIP_TRY(ctx, hipGraphLaunch(g.exec, ctx->stream));
         ctx->x = g.post_x; ctx->x_alt = g.post_x_alt; ctx->v = g.post_v; ctx->v_alt = g.post_v_alt;
         ctx->xs_cur = g.post_xs_cur;
-        ctx->cells_hold_keys = ctx->cells_hold_keys || g.sorts;
-        ctx->order_sorted = ctx->order_sorted || g.sorts;
+        ctx->cells_hold_keys = g.post_hold;
+        ctx->order_sorted = g.post_ord;
         return PBF_OK;
     }
     // capture this variant once
     const bool hold = ctx->cells_hold_keys, ord = ctx->order_sorted;
     const Rotation before = rotation_of(ctx);
     HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_step(ctx, pause);
+    int rc = PBF_OK;
+    for (uint32_t s = 0; s < steps && rc == PBF_OK; ++s) rc = enqueue_step(ctx, pause);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
     if (rc != PBF_OK) {
@@ -391,19 +395,38 @@ int step_graph(pbf_ctx* ctx, uint32_t pause) {
     if (ec != hipSuccess) return fail(ctx, PBF_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
     pbf::GraphEntry g;
     g.key = key;
-    g.sorts = (key & 4u) != 0;
+    g.steps = steps;
     const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) return fail(ctx, PBF_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
     g.post_x = ctx->x; g.post_x_alt = ctx->x_alt; g.post_v = ctx->v; g.post_v_alt = ctx->v_alt;
     g.post_xs_cur = ctx->xs_cur;
+    g.post_hold = ctx->cells_hold_keys;
+    g.post_ord = ctx->order_sorted;
     ctx->graphs.push_back(g);
     // the capture recorded the work without running it: rewind and launch
     ctx->x = before.x; ctx->x_alt = before.x_alt; ctx->v = before.v; ctx->v_alt = before.v_alt;
     ctx->xs_cur = before.xs_cur;
     ctx->cells_hold_keys = hold;
     ctx->order_sorted = ord;
-    return step_graph(ctx, pause);
+    return step_graph(ctx, pause, steps);
+}
+
+// What every step checks first: PBF_OK to go on, or the code to return
+// (*skip with PBF_OK when there is nothing to step).
+int step_ready(pbf_ctx* ctx, bool* skip) {
+    *skip = false;
+    if (!ctx) return PBF_E_INVALID;
+    if (!ctx->have_state) return fail(ctx, PBF_E_STATE, "pbf_step before pbf_set_state");
+    if (ctx->n == 0) {
+        *skip = true;
+        return PBF_OK;
+    }
+    if (ctx->x_base == nullptr) {
+        ctx->x_base = ctx->x;
+        ctx->v_base = ctx->v;
+    }
+    return PBF_OK;
 }
 
 }  // namespace
@@ -660,19 +683,25 @@ int pbf_set_collision_quads(pbf_ctx* ctx, uint32_t scene, const pbf_quad* quads,
 }
 
 int pbf_step(pbf_ctx* ctx, uint32_t pause) {
-    if (!ctx) return PBF_E_INVALID;
-    if (!ctx->have_state) return fail(ctx, PBF_E_STATE, "pbf_step before pbf_set_state");
-    if (ctx->n == 0) return PBF_OK;
-    if (ctx->x_base == nullptr) {
-        ctx->x_base = ctx->x;
-        ctx->v_base = ctx->v;
-    }
-    if (ctx->use_graph && !ctx->profiling) return step_graph(ctx, pause);
+    bool skip;
+    const int rc = step_ready(ctx, &skip);
+    if (rc != PBF_OK || skip) return rc;
+    if (ctx->use_graph && !ctx->profiling) return step_graph(ctx, pause, 1);
     return enqueue_step(ctx, pause);
 }
 
 int pbf_run(pbf_ctx* ctx, uint32_t steps, uint32_t pause) {
-    for (uint32_t s = 0; s < steps; ++s) {
+    if (steps == 0) return ctx ? PBF_OK : PBF_E_INVALID;
+    bool skip;
+    const int rc0 = step_ready(ctx, &skip);
+    if (rc0 != PBF_OK || skip) return rc0;
+    uint32_t s = 0;
+    if (ctx->use_graph && !ctx->profiling)
+        for (; steps - s >= pbf::kRunGraphSteps; s += pbf::kRunGraphSteps) {
+            const int rc = step_graph(ctx, pause, pbf::kRunGraphSteps);
+            if (rc) return rc;
+        }
+    for (; s < steps; ++s) {
         const int rc = pbf_step(ctx, pause);
         if (rc) return rc;
     }

@@ -68,11 +68,16 @@ struct EventPair {
 // host applies the rotation the captured step performs when it replays it.
 struct GraphEntry {
     uint32_t key = 0;
-    bool sorts = false;
+    uint32_t steps = 1;   // consecutive substeps the graph holds
     hipGraphExec_t exec = nullptr;
     float4 *post_x = nullptr, *post_x_alt = nullptr, *post_v = nullptr, *post_v_alt = nullptr;
     int post_xs_cur = 0;
+    bool post_hold = false, post_ord = false;   // cells_hold_keys, order_sorted after them
 };
+
+// pbf_run replays its substeps in graphs of this many: the substeps of a
+// group follow each other inside one graph, with no graph launch between.
+constexpr uint32_t kRunGraphSteps = 4;
 
 }  // namespace pbf
 

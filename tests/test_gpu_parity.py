@@ -127,6 +127,40 @@ def test_tiny_counts(n):
     sim.close()
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_random_scenes(seed):
+    """Scenes drawn at random (count, h, box, grid size with and without
+    folding, solver iterations, gravity vector, walls, initial velocities,
+    particles outside the box, pause pattern) give the oracle's structure and
+    state bit for bit over a few substeps."""
+    rng = np.random.default_rng(7000 + seed)
+    n = int(rng.integers(40, 3000))
+    h = float(rng.choice([0.05, 0.1, 0.25]))
+    extent = rng.uniform(8 * h, 30 * h, 3)
+    lo = rng.uniform(-2.0, 2.0, 3)
+    gbs = [None, 64, 1000, 4096, 12345, 1 << 16][seed]
+    iters = int(rng.integers(1, 7))
+    p = pbf_amd.physical_params(n, h, lo, lo + extent, iters=iters, grid_buf_size=gbs,
+                                xsph_c=float(rng.uniform(0.0, 0.05)), vort_eps=float(rng.uniform(0.0, 1e-3)))
+    g = rng.normal(0.0, 5.0, 3) if seed % 2 else None
+    opts = pbf_amd.default_options(walls=bool(seed % 3), gravity=g)
+    # a block at about twice the rest density in one corner and a sparse
+    # cloud, a few particles outside the box
+    dense = n * 2 // 3
+    side = np.minimum(extent, (dense / 16.0) ** (1.0 / 3.0) * h)
+    x = np.zeros((n, 4), np.float32)
+    x[:dense, :3] = lo + rng.uniform(0.0, 1.0, (dense, 3)) * side
+    x[dense:, :3] = lo + rng.uniform(-0.1, 1.1, (n - dense, 3)) * extent
+    v = np.zeros((n, 4), np.float32)
+    v[:, :3] = rng.normal(0.0, 0.5, (n, 3))
+    scene = pbf_amd.Scene(p, opts, x, v)
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 4, pauses=list(rng.integers(0, 2, 4)))
+    check_structure(sim, ref)
+    check_state(sim, ref, scene)
+    sim.close()
+
+
 def test_pause_semantics():
     scene = pbf_amd.config("dam4k", seed=9)
     sim, ref = make_pair(scene)

@@ -12,7 +12,8 @@
 //                              derivative of the binary32 trajectory the GPU
 //                              computes (its values equal the oracle's bit for
 //                              bit, because every operation is in the oracle's
-//                              order and the build does no FMA contraction).
+//                              order, fused by t_fma where the oracle calls
+//                              fmaf, and the build does no other contraction).
 // With the structure frozen the function is smooth: every kernel is C1 at
 // the r = h cutoff, and the remaining branches (the density mask of XSPH and
 // vorticity, eta = 0) are taken on the primal values.
@@ -64,6 +65,16 @@ inline Dual<float> t_sqrt(Dual<float> x) {
     const float s = sqrtf(x.v);
     return {s, x.d / (2.0 * (double)s)};
 }
+
+// The oracle's fmaf at the same points: one rounding in the value's own
+// precision; a dual's tangent is that of a * b + c.
+inline double t_fma(double a, double b, double c) { return fma(a, b, c); }
+inline float t_fma(float a, float b, float c) { return fmaf(a, b, c); }
+template <class P> inline Dual<P> t_fma(Dual<P> a, Dual<P> b, Dual<P> c) {
+    return {t_fma(a.v, b.v, c.v), a.d * (double)b.v + (double)a.v * b.d + c.d};
+}
+// r^2 of a pair as the oracle's dist2
+template <class T> inline T dist2(const T* r) { return t_fma(r[2], r[2], t_fma(r[1], r[1], r[0] * r[0])); }
 
 // Wall noise: sin in FP64; the binary32 oracle's own sine for Dual<float>.
 inline double t_sin(double x) { return sin(x); }
@@ -221,7 +232,7 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                 for_friends(fr, n, i, 0.6f, [&](uint32_t j) {
                     const T* pj = xs[j].a;
                     const T r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                    const T r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                    const T r2 = dist2(r);
                     if (val(r2) < val(h2)) {
                         const T rl = pair_rl(r2);
                         const T hr = h - rl;
@@ -230,8 +241,8 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                         const T t = sp * inv;
                         const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                         const T d = h2 - r2;
-                        dens += d * d * d;
-                        gsum += gg[0] * gg[0] + gg[1] * gg[1] + gg[2] * gg[2];
+                        dens = t_fma(d * d, d, dens);
+                        gsum = t_fma(gg[2], gg[2], t_fma(gg[1], gg[1], t_fma(gg[0], gg[0], gsum)));
                         gi[0] += gg[0]; gi[1] += gg[1]; gi[2] += gg[2];
                     }
                 });
@@ -250,7 +261,7 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                 for_friends(fr, n, i, 0.5f, [&](uint32_t j) {
                     const T* pj = xs[j].a;
                     const T r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                    const T r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                    const T r2 = dist2(r);
                     if (val(r2) < val(h2)) {
                         const T rl = pair_rl(r2);
                         const T hr = h - rl;
@@ -260,9 +271,9 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                         const T d = h2 - r2;
                         const T p6 = d * d * d;
                         const T ratio = p6 * inv_q;
-                        const T scorr = stk * ratio * ratio * ratio * ratio;
-                        const T co = lam[i] + lam[j] + scorr;
-                        sum[0] += co * gg[0]; sum[1] += co * gg[1]; sum[2] += co * gg[2];
+                        const T r4 = ratio * ratio;
+                        const T co = t_fma(stk, r4 * r4, lam[i] + lam[j]);
+                        for (int k = 0; k < 3; ++k) sum[k] = t_fma(co, gg[k], sum[k]);
                     }
                 });
                 T fut[3];
@@ -309,7 +320,7 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                 const T* pj = xs[j].a;
                 const T* vj = v[j].a;
                 const T r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                const T r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                const T r2 = dist2(r);
                 if (val(r2) < val(h2) && fabs(val(rho[j])) > (double)1e-8f) {
                     const T u[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
                     const T d = h2 - r2;
@@ -318,11 +329,11 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                     const T inv = val(rl) > 0.0 ? one / rl : zero;
                         const T t = inv * hr * hr;
                         const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
-                    o[0] += u[1] * gg[2] - u[2] * gg[1];
-                    o[1] += u[2] * gg[0] - u[0] * gg[2];
-                    o[2] += u[0] * gg[1] - u[1] * gg[0];
+                    o[0] += t_fma(u[1], gg[2], -(u[2] * gg[1]));
+                    o[1] += t_fma(u[2], gg[0], -(u[0] * gg[2]));
+                    o[2] += t_fma(u[0], gg[1], -(u[1] * gg[0]));
                     const T p6 = d * d * d;
-                    vs[0] += u[0] * p6; vs[1] += u[1] * p6; vs[2] += u[2] * p6;
+                    for (int k = 0; k < 3; ++k) vs[k] = t_fma(u[k], p6, vs[k]);
                 }
             });
             for (int k = 0; k < 3; ++k) {
@@ -340,7 +351,7 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
             for_friends(fr, n, i, 0.5f, [&](uint32_t j) {
                 const T* pj = xs[j].a;
                 const T r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                const T r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                const T r2 = dist2(r);
                 if (val(r2) < val(h2) && fabs(val(rho[j])) > (double)1e-8f) {
                     const T rl = pair_rl(r2);
                     const T hr = h - rl;
@@ -349,7 +360,7 @@ void substep(const pbfo_cfg& c, const Frozen& f, uint32_t s, const T* coef, Stat
                         const T gg[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const T* oj = om[j].a;
                     const T wl = pair_rl(oj[0] * oj[0] + oj[1] * oj[1] + oj[2] * oj[2]);
-                    eta[0] += wl * gg[0]; eta[1] += wl * gg[1]; eta[2] += wl * gg[2];
+                    for (int k = 0; k < 3; ++k) eta[k] = t_fma(wl, gg[k], eta[k]);
                 }
             });
             for (int k = 0; k < 3; ++k) eta[k] *= minus1 * spiky;

@@ -3,9 +3,10 @@
  *
  * TEST INFRASTRUCTURE ONLY (see pbf_oracle.h).  Built with
  * -ffp-contract=off -fno-fast-math so every expression below is evaluated as
- * written in IEEE binary32.  Parallel over particles with OpenMP; each
- * particle's sums are sequential in the reference's friend order, so results
- * do not depend on the thread count.
+ * written in IEEE binary32 (the fused multiply-adds are explicit fmaf
+ * calls).  Parallel over particles with OpenMP; each particle's sums are
+ * sequential in the reference's friend order, so results do not depend on
+ * the thread count.
  */
 #include "pbf_oracle.h"
 
@@ -202,6 +203,21 @@ static inline void cross3(const float *a, const float *b, float *o)
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+/* Fused multiply-adds at fixed points, the same ones the GPU kernels fuse:
+ * the squared distance of a pair, fmaf(rz, rz, fmaf(ry, ry, rx * rx)), and
+ * each product that a sweep adds into a sum.  fmaf rounds once, as the
+ * hardware FMA does; every other expression here stays unfused.  The
+ * reference compiles its kernels with -cl-mad-enable (src/Simulation.cpp:160),
+ * which lets its compiler fuse any of these. */
+static inline float dist2(const float *r) { return fmaf(r[2], r[2], fmaf(r[1], r[1], r[0] * r[0])); }
+/* a x b with each component a fused difference of products */
+static inline void cross3_fused(const float *a, const float *b, float *o)
+{
+    o[0] = fmaf(a[1], b[2], -(a[2] * b[1]));
+    o[1] = fmaf(a[2], b[0], -(a[0] * b[2]));
+    o[2] = fmaf(a[0], b[1], -(a[1] * b[0]));
+}
+
 
 /* build_friends_list.cl:23-103 */
 void pbfo_friends(const pbfo_cfg *c, const float *xs, const uint32_t *cells, uint32_t *fr)
@@ -229,7 +245,7 @@ void pbfo_friends(const pbfo_cfg *c, const float *xs, const uint32_t *cells, uin
                         if (j == i) continue;
                         const float *pj = F4(xs, j);
                         float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                        const float r2 = dot3(r, r);
+                        const float r2 = dist2(r);
                         /* build_friends_list.cl:70 skips r^2 >= h^2; a NaN
                          * r^2 (relaxed math there: undefined) is skipped too */
                         if (!(r2 < h2)) continue;
@@ -268,7 +284,7 @@ void pbfo_scaling(const pbfo_cfg *c, const float *xs, const uint32_t *fr, float 
             for (int f = 0; f < cnt[q]; ++f) {
                 const float *pj = F4(xs, FRIEND(fr, n, q, f, i));
                 const float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                const float r2 = dot3(r, r);
+                const float r2 = dist2(r);
                 if (r2 < h2) {
                     const float rl = pair_rl(r2);
                     const float hr = h - rl;
@@ -277,8 +293,8 @@ void pbfo_scaling(const pbfo_cfg *c, const float *xs, const uint32_t *fr, float 
                     const float t = s * inv;
                     const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const float d = h2 - r2;
-                    dens += d * d * d;
-                    gsum += dot3(g, g);
+                    dens = fmaf(d * d, d, dens);
+                    gsum = fmaf(g[2], g[2], fmaf(g[1], g[1], fmaf(g[0], g[0], gsum)));
                     gi[0] += g[0]; gi[1] += g[1]; gi[2] += g[2];
                 }
             }
@@ -427,7 +443,7 @@ void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const u
             for (int f = 0; f < cnt[q]; ++f) {
                 const float *pj = F4(xs, FRIEND(fr, n, q, f, i));
                 const float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                const float r2 = dot3(r, r);
+                const float r2 = dist2(r);
                 if (r2 < h2) {
                     const float rl = pair_rl(r2);
                     const float hr = h - rl;
@@ -437,9 +453,10 @@ void pbfo_delta_hits(const pbfo_cfg *c, const float *x, const float *xs, const u
                     const float d = h2 - r2;
                     const float p6 = d * d * d;
                     const float ratio = p6 * inv_q;
-                    const float scorr = K * ratio * ratio * ratio * ratio;
-                    const float coef = pi[3] + pj[3] + scorr;
-                    sum[0] += coef * g[0]; sum[1] += coef * g[1]; sum[2] += coef * g[2];
+                    /* lambda_i + lambda_j + s_corr, s_corr = K ratio^4 as (ratio^2)^2 */
+                    const float r4 = ratio * ratio;
+                    const float coef = fmaf(K, r4 * r4, pi[3] + pj[3]);
+                    for (int k = 0; k < 3; ++k) sum[k] = fmaf(coef, g[k], sum[k]);
                 }
             }
         }
@@ -518,7 +535,7 @@ void pbfo_viscosity(const pbfo_cfg *c, const float *xs, const float *v_in, const
                 const uint32_t j = FRIEND(fr, n, q, f, i);
                 const float *pj = F4(xs, j), *vj = F4(v_in, j);
                 const float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                const float r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                const float r2 = dist2(r);
                 if (r2 < h2 && fabsf(pj[3]) > 1e-8f) {
                     const float vij[3] = {vj[0] - vi[0], vj[1] - vi[1], vj[2] - vi[2]};
                     const float d = h2 - r2;
@@ -528,10 +545,10 @@ void pbfo_viscosity(const pbfo_cfg *c, const float *xs, const float *v_in, const
                     const float t = inv * hr * hr;
                     const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     float cr[3];
-                    cross3(vij, g, cr);
+                    cross3_fused(vij, g, cr);
                     om[0] += cr[0]; om[1] += cr[1]; om[2] += cr[2];
                     const float p6 = d * d * d;
-                    vs[0] += vij[0] * p6; vs[1] += vij[1] * p6; vs[2] += vij[2] * p6;
+                    for (int k = 0; k < 3; ++k) vs[k] = fmaf(vij[k], p6, vs[k]);
                 }
             }
         }
@@ -568,7 +585,7 @@ void pbfo_vorticity(const pbfo_cfg *c, const float *xs, float *v, const float *o
                 const uint32_t j = FRIEND(fr, n, q, f, i);
                 const float *pj = F4(xs, j);
                 const float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                const float r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                const float r2 = dist2(r);
                 if (r2 < h2 && fabsf(pj[3]) > 1e-8f) {
                     const float rl = pair_rl(r2);
                     const float hr = h - rl;
@@ -577,7 +594,7 @@ void pbfo_vorticity(const pbfo_cfg *c, const float *xs, float *v, const float *o
                     const float g[3] = {r[0] * t, r[1] * t, r[2] * t};
                     const float *oj = F4(omega, j);
                     const float wl = pair_rl(dot3(oj, oj));
-                    eta[0] += wl * g[0]; eta[1] += wl * g[1]; eta[2] += wl * g[2];
+                    for (int k = 0; k < 3; ++k) eta[k] = fmaf(wl, g[k], eta[k]);
                 }
             }
         }

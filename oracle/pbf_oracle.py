@@ -52,11 +52,24 @@ def build() -> Path:
 _L = None
 
 
+def _cpu_has_fma() -> bool:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("flags"):
+                    return " fma " in line + " "
+    except OSError:
+        pass
+    return True   # no cpuinfo to read: let the load decide
+
+
 def lib() -> ctypes.CDLL:
     global _L
     if _L is None:
         if not LIB_PATH.exists():
             build()
+        if not _cpu_has_fma():
+            raise RuntimeError("the oracle is built with -mfma (x86-64 FMA3); this CPU lacks it")
         L = ctypes.CDLL(str(LIB_PATH))
         vp = ctypes.c_void_p
         cp = ctypes.POINTER(Cfg)

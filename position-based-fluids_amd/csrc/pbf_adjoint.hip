@@ -79,7 +79,7 @@ struct Pair {
 __device__ __forceinline__ Pair pair_of(const float4& a, const float4& b, const SceneConsts& c) {
     Pair g;
     g.r = {a.x - b.x, a.y - b.y, a.z - b.z};
-    g.r2 = dot(g.r, g.r);
+    g.r2 = dist2(g.r.x, g.r.y, g.r.z);   // as the forward sweeps
     g.in = g.r2 < c.h2;
     const PairLen L = pair_len(g.r2);   // as the forward sweeps: 0 and 0 for a coincident pair
     g.q = L.rl;
@@ -308,8 +308,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_transpose_all(
                 for (uint32_t i = b.x; i <= e; ++i) {
                     if (i == m) continue;
                     const float4 q = xs[i];
-                    const float rx = q.x - p.x, ry = q.y - p.y, rz = q.z - p.z;
-                    if (!(rx * rx + ry * ry + rz * rz < c.h2)) continue;
+                    // the forward build's friend test (the sign of r does not change r^2)
+                    if (!(dist2(q.x - p.x, q.y - p.y, q.z - p.z) < c.h2)) continue;
                     int m05, m06;
                     const uint32_t pi = cnt[i];
                     circles_processed<1, 2>(pi, &m05);

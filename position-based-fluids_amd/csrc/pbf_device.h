@@ -205,6 +205,19 @@ __device__ __forceinline__ uint32_t lane_mask(bool keep) {
     return m;
 }
 __device__ __forceinline__ float masked(float x, uint32_t m) { return __uint_as_float(__float_as_uint(x) & m); }
+// a where m is all ones, else b (one bit select): a sum that a skipped pair
+// leaves as it is.
+__device__ __forceinline__ float picked(float a, float b, uint32_t m) {
+    return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
+}
+
+// Fused multiply-adds at fixed points, the same in the oracle: the squared
+// distance of a pair is fmaf(rz, rz, fmaf(ry, ry, rx * rx)), and the sweeps
+// fold each pair's terms into their sums with one fmaf where a product is
+// added.  Everything else stays unfused (the build has contraction off).
+// The reference compiles its kernels with -cl-mad-enable, which lets its
+// compiler fuse any of these.
+__device__ __forceinline__ float dist2(float rx, float ry, float rz) { return fmaf(rz, rz, fmaf(ry, ry, rx * rx)); }
 
 __device__ __forceinline__ int circle_count(uint32_t packed, int q) { return (packed >> (6 * q)) & 63; }
 

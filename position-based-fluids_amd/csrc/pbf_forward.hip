@@ -192,7 +192,7 @@ struct LambdaAcc {
     // loop, which keeps the next gathers in flight.
     __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c, bool valid) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r2 = dist2(rx, ry, rz);
         const uint32_t in = lane_mask(valid && r2 < c.h2);
         const PairLen L = pair_len_sweep(r2);   // r/|r| = r * (1/|r|); 0 for a coincident pair
         const float hr = c.h - L.rl;
@@ -200,8 +200,8 @@ struct LambdaAcc {
         const float t = s * L.inv;
         const float ax = masked(rx * t, in), ay = masked(ry * t, in), az = masked(rz * t, in);
         const float d = masked(c.h2 - r2, in);
-        dens += d * d * d;
-        gsum += ax * ax + ay * ay + az * az;
+        dens = fmaf(d * d, d, dens);
+        gsum = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, gsum)));
         gx += ax;
         gy += ay;
         gz += az;
@@ -273,8 +273,7 @@ __device__ __forceinline__ int wave_max(int v) {
 // of thresholds circ_t[k] <= r^2; the thresholds are found on the host by
 // evaluating the same expression.
 __device__ __forceinline__ int friend_circle(const float4& p, const float4& q, const SceneConsts& c) {
-    const float rx = p.x - q.x, ry = p.y - q.y, rz = p.z - q.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
     const int circ =
         (int)(r2 >= c.circ_t[0]) + (int)(r2 >= c.circ_t[1]) + (int)(r2 >= c.circ_t[2]) + (int)(r2 >= c.circ_t[3]);
     return (int)((uint32_t)circ | ~lane_mask(r2 < c.h2));   // -1 out of range, computed without a branch
@@ -830,8 +829,8 @@ struct DeltaAcc {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     __device__ __forceinline__ void add(const float4& p, const float4& pj, const SceneConsts& c, bool valid) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const uint32_t in = lane_mask(valid && r2 < c.h2);   // selected to +0 otherwise (as in LambdaAcc)
+        const float r2 = dist2(rx, ry, rz);
+        const uint32_t in = lane_mask(valid && r2 < c.h2);   // left out otherwise (as in LambdaAcc)
         const PairLen L = pair_len_sweep(r2);
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
@@ -839,11 +838,11 @@ struct DeltaAcc {
         const float d = c.h2 - r2;
         const float p6 = d * d * d;
         const float ratio = p6 * c.inv_poly6_q;
-        const float scorr = c.st_k * ratio * ratio * ratio * ratio;
-        const float coef = p.w + pj.w + scorr;
-        sx += masked(coef * ax, in);
-        sy += masked(coef * ay, in);
-        sz += masked(coef * az, in);
+        const float r4 = ratio * ratio;
+        const float coef = fmaf(c.st_k, r4 * r4, p.w + pj.w);   // lambda_i + lambda_j + s_corr
+        sx = picked(fmaf(coef, ax, sx), sx, in);
+        sy = picked(fmaf(coef, ay, sy), sy, in);
+        sz = picked(fmaf(coef, az, sz), sz, in);
     }
 };
 
@@ -908,8 +907,8 @@ struct ViscAcc {
     __device__ __forceinline__ void add(const float4& p, const float4& vi, const float4& pj, const float4& vj,
                                         const SceneConsts& c, bool valid) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        // a skipped pair's contributions are selected to +0 (as in LambdaAcc)
+        const float r2 = dist2(rx, ry, rz);
+        // a skipped pair leaves the sums as they are (as in LambdaAcc)
         const uint32_t in = lane_mask(valid && r2 < c.h2 && fabsf(pj.w) > 1e-8f);
         const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
         const float d = c.h2 - r2;
@@ -917,13 +916,13 @@ struct ViscAcc {
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
-        ox += masked(uy * az - uz * ay, in);
-        oy += masked(uz * ax - ux * az, in);
-        oz += masked(ux * ay - uy * ax, in);
+        ox = picked(ox + fmaf(uy, az, -(uz * ay)), ox, in);   // (v_j - v_i) x grad
+        oy = picked(oy + fmaf(uz, ax, -(ux * az)), oy, in);
+        oz = picked(oz + fmaf(ux, ay, -(uy * ax)), oz, in);
         const float p6 = d * d * d;
-        vsx += masked(ux * p6, in);
-        vsy += masked(uy * p6, in);
-        vsz += masked(uz * p6, in);
+        vsx = picked(fmaf(ux, p6, vsx), vsx, in);
+        vsy = picked(fmaf(uy, p6, vsy), vsy, in);
+        vsz = picked(fmaf(uz, p6, vsz), vsz, in);
     }
 };
 
@@ -974,16 +973,16 @@ struct VortAcc {
     __device__ __forceinline__ void add(const float4& p, const float4& pj, const float4& oj, const SceneConsts& c,
                                         bool valid) {
         const float rx = p.x - pj.x, ry = p.y - pj.y, rz = p.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r2 = dist2(rx, ry, rz);
         const uint32_t in = lane_mask(valid && r2 < c.h2 && fabsf(pj.w) > 1e-8f);
         const PairLen L = pair_len_sweep(r2);
         const float hr = c.h - L.rl;
         const float t = L.inv * hr * hr;
         const float ax = rx * t, ay = ry * t, az = rz * t;
         const float wl = oj.w;   // |omega_j| (fast_length, apply_vorticity.cl:63), stored by k_viscosity
-        ex += masked(wl * ax, in);
-        ey += masked(wl * ay, in);
-        ez += masked(wl * az, in);
+        ex = picked(fmaf(wl, ax, ex), ex, in);
+        ey = picked(fmaf(wl, ay, ey), ey, in);
+        ez = picked(fmaf(wl, az, ez), ez, in);
     }
 };
 

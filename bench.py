@@ -50,16 +50,16 @@ STAGE_BYTES = {
 # and indentation left out): with any other code the line reports traffic as
 # null.
 PMC_TRAFFIC = {
-    "source_sha16": "2b5f089b1d08bb71",
+    "source_sha16": "f5d1cd2fda632585",
     "bytes_per_launch": {
-        "applyViscosity": 215139164,
-        "applyVorticity": 220116403,
-        "buildFriendsList": 932116895,
-        "computeDelta": 206429283,
-        "computeScaling": 204482001,
-        "onesweep": 17694631,
-        "predictPositions": 43432847,
-        "sortParticles": 96787769,
+        "applyViscosity": 215775570,
+        "applyVorticity": 231688365,
+        "buildFriendsList": 933195504,
+        "computeDelta": 205182387,
+        "computeScaling": 203778817,
+        "onesweep": 17371751,
+        "predictPositions": 43434945,
+        "sortParticles": 96768047,
     },
 }
 

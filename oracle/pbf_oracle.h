@@ -8,9 +8,10 @@
  *
  * Every stage restates one kernel of the reference (WorkingXXX/
  * position-based-fluids, assets/kernels/<stage>.cl) driven in the order of
- * src/Simulation.cpp:554-621, with IEEE single precision, no FMA contraction
- * (built with -ffp-contract=off) and the operation order of the reference's
- * expressions.  Where the reference relies on -cl-fast-relaxed-math or has
+ * src/Simulation.cpp:554-621, with IEEE single precision, no implicit FMA
+ * contraction (built with -ffp-contract=off; explicit fmaf only where the GPU
+ * kernels fuse the same operations) and the operation order of the
+ * reference's expressions.  Where the reference relies on -cl-fast-relaxed-math or has
  * undefined behaviour, the choice made here is stated at the function; one
  * applies to every friend sweep: the vector r / |r| (compute_scaling.cl:83,
  * compute_delta.cl:144, apply_viscosity.cl:65, apply_vorticity.cl) is

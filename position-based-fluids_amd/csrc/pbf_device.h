@@ -1,8 +1,9 @@
 // pbf_device.h -- device-side helpers shared by the PBF kernels.
 //
-// Everything here is evaluated in IEEE binary32 with no FMA contraction (the
-// library is compiled with -ffp-contract=off), in the operation order of the
-// reference kernels, so that results are reproducible across launch shapes.
+// Everything here is evaluated in IEEE binary32 with no implicit FMA
+// contraction (the library is compiled with -ffp-contract=off; the fmaf calls
+// are the only fused operations), in the operation order of the reference
+// kernels, so that results are reproducible across launch shapes.
 #pragma once
 
 #include <hip/hip_runtime.h>

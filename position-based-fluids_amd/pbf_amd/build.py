@@ -30,8 +30,9 @@ HIP_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    # IEEE binary32 as written: no FMA contraction (results reproducible and
-    # identical to the CPU oracle's operation order)
+    # IEEE binary32 as written: no implicit FMA contraction, only the fmaf
+    # calls in the source (results reproducible and identical to the CPU
+    # oracle's, which fuses at the same points)
     "-ffp-contract=off",
     # never emit packed-fp32 VALU ops
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -3,8 +3,8 @@
 Integer structure (cell keys, sort permutation, cells, friends lists) must
 match bit for bit.  Floating-point state must agree within 1e-5*h in
 positions (1e-5*h/dt in velocities); both sides evaluate the same binary32
-operations in the same order without contraction, so the state is also
-checked for bitwise equality.
+operations in the same order, fusing multiply-adds at the same points only,
+so the state is also checked for bitwise equality.
 """
 import numpy as np
 import pytest

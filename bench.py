@@ -200,6 +200,19 @@ def cpu_baseline(scene, budget_s: float = 15.0, runs: int = 3) -> dict:
     }
 
 
+CAPTURE_STEPS = 6
+
+
+def capture_graphs(sim) -> None:
+    """Runs CAPTURE_STEPS substeps that capture every graph a later run
+    replays from the state they leave: one-substep graphs of both buffer
+    rotations (pbf_run's remainder), then a group of four (pbf_run groups
+    four substeps to a graph), so that no capture falls inside a timed run
+    of any length.  Both rotations recur every two substeps."""
+    sim.run(2)
+    sim.run(4)
+
+
 def full_run(scene, sizes: dict | None) -> dict:
     """All 500 substeps of the workload from its initial state, replayed as
     HIP graphs (the default launch shapes, or the tuned sizes), with the
@@ -248,7 +261,7 @@ def batched_gain(steps: int, warmup: int, seed: int, tune: bool) -> dict:
     batch.run(warmup)
     sizes = batch.autotune(2) if tune else None
     batch.set_graph(True)
-    batch.run(4)                    # captures the graphs the timed run replays
+    capture_graphs(batch)
     batch.sync()
     t0 = time.perf_counter()
     batch.run(steps)
@@ -269,7 +282,7 @@ def batched_gain(steps: int, warmup: int, seed: int, tune: bool) -> dict:
     for sim in singles:
         sim.set_graph(True)
         sim.run(warmup)
-        sim.run(4)                  # as the batch: its graphs captured before timing
+        capture_graphs(sim)         # as the batch
         sim.sync()
         t0 = time.perf_counter()
         sim.run(steps)
@@ -281,7 +294,7 @@ def batched_gain(steps: int, warmup: int, seed: int, tune: bool) -> dict:
     def loop_time(graph: bool) -> float:
         for sim in singles:
             sim.set_graph(graph)
-        for _ in range(warmup + 4):   # the batch's timed substeps
+        for _ in range(warmup + CAPTURE_STEPS):   # the batch's timed substeps
             for sim in singles:
                 sim.Step()
         for sim in singles:
@@ -696,7 +709,7 @@ def main() -> int:
     sizes = sim.autotune(4) if not args.no_autotune else sim.stats()["workgroup_size"]
     # the timed region: substeps replayed as HIP graphs, no events between launches
     sim.set_graph(True)
-    sim.run(4)                      # captures the graphs the timed run replays
+    capture_graphs(sim)
     sim.sync()
     start = sim.get_state()
     t0 = time.perf_counter()
@@ -790,7 +803,7 @@ def main() -> int:
             "solver_iterations": iters,
             "grid_buf_size": int(scene.params.gridBufSize),
             "sort_passes": passes,
-            "substeps_before_timing": args.warmup,
+            "substeps_before_timing": args.warmup + CAPTURE_STEPS,
             "workgroup_sizes": sizes,
         },
         "roofline": {

@@ -52,14 +52,14 @@ STAGE_BYTES = {
 PMC_TRAFFIC = {
     "source_sha16": "f5d1cd2fda632585",
     "bytes_per_launch": {
-        "applyViscosity": 215775570,
-        "applyVorticity": 231688365,
-        "buildFriendsList": 933195504,
-        "computeDelta": 205182387,
-        "computeScaling": 203778817,
-        "onesweep": 17371751,
-        "predictPositions": 43434945,
-        "sortParticles": 96768047,
+        "applyViscosity": 220147697,
+        "applyVorticity": 221264322,
+        "buildFriendsList": 944571338,
+        "computeDelta": 206793692,
+        "computeScaling": 205410462,
+        "onesweep": 17673322,
+        "predictPositions": 43423758,
+        "sortParticles": 97364246,
     },
 }
 

@@ -161,6 +161,35 @@ def test_random_scenes(seed):
     sim.close()
 
 
+# pairs whose |r|^2 with fmaf and unfused lie on either side of h^2 = 0.1f^2
+_STRADDLING_PAIRS = [
+    ((0.2739469110965729, 0.3455261290073395, 0.5627720355987549),
+     (0.2296944111585617, 0.379925400018692, 0.4799565374851227)),
+    ((0.4090348780155182, 0.3165309727191925, 0.20136530697345734),
+     (0.42549607157707214, 0.22700658440589905, 0.24277077615261078)),
+    ((0.4496609568595886, 0.23794008791446686, 0.21546831727027893),
+     (0.46433621644973755, 0.1931574046611786, 0.12726883590221405)),
+]
+
+
+@pytest.mark.parametrize("k", range(len(_STRADDLING_PAIRS)))
+def test_friend_test_at_the_fused_distance(k):
+    """A paused substep from rest keeps positions exact, so the friends build
+    sees each pair as placed: its friend test agrees with the oracle on pairs
+    whose fused and unfused |r|^2 straddle h^2."""
+    a, b = _STRADDLING_PAIRS[k]
+    x = np.zeros((2, 4), np.float32)
+    x[0, :3], x[1, :3] = a, b
+    p = pbf_amd.physical_params(2, 0.1, (0, 0, 0), (1.0, 1.0, 1.0))
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(walls=False), x, np.zeros_like(x))
+    sim, ref = make_pair(scene)
+    advance(sim, ref, 1, pauses=[1])
+    st = check_structure(sim, ref)
+    sim.close()
+    fr = st["friends"]
+    assert int(fr[:10].sum()) == [0, 2, 0][k]   # pairs 0 and 2 are not friends, pair 1 is
+
+
 def test_pause_semantics():
     scene = pbf_amd.config("dam4k", seed=9)
     sim, ref = make_pair(scene)

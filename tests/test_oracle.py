@@ -5,6 +5,7 @@ the restatement where the mathematics or an independent computation does.
 """
 import ctypes
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -183,6 +184,36 @@ def _structure(scene):
     return c, xs, ks, cells, fr
 
 
+def _round_f32(fr):
+    """A Fraction rounded to binary32, to nearest, ties to even."""
+    if fr == 0:
+        return np.float32(0.0)
+    c = np.float32(float(fr))
+    cands = (c, np.nextafter(c, np.float32(np.inf)), np.nextafter(c, np.float32(-np.inf)))
+    return min(cands, key=lambda y: (abs(Fraction(float(y)) - fr), int(np.float32(y).view(np.uint32)) & 1))
+
+
+def _fmaf(a, b, c):
+    """fmaf in exact rational arithmetic: a * b + c rounded once."""
+    return _round_f32(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+def fused_r2(r, near):
+    """|r|^2 of binary32 rows as the oracle and the kernels take it,
+    fmaf(rz, rz, fmaf(ry, ry, rx * rx)).  The sums are taken in float64 (a
+    48-bit product plus a binary32 need not fit its 53 bits); the rows within
+    16 ulps of `near` (the value compared against) are redone in exact
+    rational arithmetic, so every comparison with `near` is exact."""
+    r = np.asarray(r, np.float32).reshape(-1, 3)
+    a = r[:, 0] * r[:, 0]
+    s1 = (r[:, 1].astype(np.float64) ** 2 + a.astype(np.float64)).astype(np.float32)
+    out = (r[:, 2].astype(np.float64) ** 2 + s1.astype(np.float64)).astype(np.float32)
+    ulp = float(np.spacing(np.float32(near)))
+    for k in np.nonzero(np.abs(out.astype(np.float64) - float(near)) <= 16 * ulp)[0]:
+        out[k] = _fmaf(r[k, 2], r[k, 2], _fmaf(r[k, 1], r[k, 1], a[k]))
+    return out
+
+
 def test_friends_equal_brute_force_and_order(oracle):
     """Friend set == {j != i : |r|^2 < h^2}; circle rule; scan order."""
     scene = pbf_amd.dam_break(12, 10, 9, 0.1, (1.2, 1.2, 1.2), seed=21)
@@ -194,7 +225,7 @@ def test_friends_equal_brute_force_and_order(oracle):
     rng = np.random.default_rng(1)
     for i in rng.choice(n, 120, replace=False):
         r = p[i] - p
-        r2 = (r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1]) + r[:, 2] * r[:, 2]
+        r2 = fused_r2(r, h2)
         want = set(np.nonzero((r2 < h2) & (np.arange(n) != i))[0].tolist())
         got = _friends_of(fr, n, i)
         flat = [j for circ in got for j in circ]
@@ -210,6 +241,52 @@ def test_friends_equal_brute_force_and_order(oracle):
                 order = ((int(cj[0]) + 1) * 9 + (int(cj[1]) + 1) * 3 + (int(cj[2]) + 1), int(j))
                 assert prev is None or order > prev, "friends not in x,y,z cell scan order"
                 prev = order
+
+
+# pairs whose |r|^2 taken with fmaf and taken unfused lie on either side of
+# h^2 = 0.1f * 0.1f (found by a search over random pairs at distance ~h)
+_STRADDLING_PAIRS = [
+    ((0.2739469110965729, 0.3455261290073395, 0.5627720355987549),
+     (0.2296944111585617, 0.379925400018692, 0.4799565374851227)),
+    ((0.4090348780155182, 0.3165309727191925, 0.20136530697345734),
+     (0.42549607157707214, 0.22700658440589905, 0.24277077615261078)),
+    ((0.4496609568595886, 0.23794008791446686, 0.21546831727027893),
+     (0.46433621644973755, 0.1931574046611786, 0.12726883590221405)),
+]
+
+
+@pytest.mark.parametrize("k", range(len(_STRADDLING_PAIRS)))
+def test_friend_test_uses_the_fused_distance(oracle, k):
+    """Two particles are friends iff fmaf(rz, rz, fmaf(ry, ry, rx*rx)) < h^2,
+    the fused form the kernels evaluate: for these pairs the unfused sum
+    falls on the other side of h^2, so a friend test or a list built from
+    the unfused form fails here."""
+    a, b = _STRADDLING_PAIRS[k]
+    x = np.zeros((2, 4), np.float32)
+    x[0, :3], x[1, :3] = a, b
+    h = 0.1
+    h2 = np.float32(np.float32(h) * np.float32(h))
+    r = (x[0, :3] - x[1, :3]).astype(np.float32)
+    unfused = np.float32(np.float32(r[0] * r[0] + r[1] * r[1]) + r[2] * r[2])
+    fused = _fmaf(r[2], r[2], _fmaf(r[1], r[1], r[0] * r[0]))
+    assert (fused < h2) != (unfused < h2)
+    p = pbf_amd.physical_params(2, h, (0, 0, 0), (1.0, 1.0, 1.0))
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(walls=False), x, np.zeros_like(x))
+    L = po.lib()
+    c = po.cfg_from_params(scene.params, scene.options)
+    keys = np.zeros(2, np.uint32)
+    L.pbfo_keys(ctypes.byref(c), po.ptr(x), po.ptr(keys))
+    ks = np.zeros(2, np.uint32)
+    perm = np.zeros(2, np.uint32)
+    L.pbfo_stable_sort(po.ptr(keys), 2, po.ptr(ks), po.ptr(perm))
+    xs = np.ascontiguousarray(x[perm])
+    cells = np.full(2 * c.grid_buf_size, 0xFFFFFFFF, np.uint32)
+    L.pbfo_cells(po.ptr(ks), 2, po.ptr(cells))
+    fr = np.zeros(2 * 255, np.uint32)
+    L.pbfo_friends(ctypes.byref(c), po.ptr(xs), po.ptr(cells), po.ptr(fr))
+    for i in range(2):
+        flat = [j for circ in _friends_of(fr, 2, i) for j in circ]
+        assert flat == ([1 - i] if fused < h2 else []), (i, flat, float(fused), float(unfused))
 
 
 def test_rest_lattice_neighbours_and_density(oracle):

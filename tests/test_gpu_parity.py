@@ -190,6 +190,27 @@ def test_friend_test_at_the_fused_distance(k):
     assert int(fr[:10].sum()) == [0, 2, 0][k]   # pairs 0 and 2 are not friends, pair 1 is
 
 
+def test_lattice_on_cell_boundaries():
+    """A lattice at spacing h/2 through the origin (h = 2^-3, every coordinate
+    exact, half of them on cell faces, negative ones in the cell of
+    truncation toward zero): a paused substep from rest keeps them there, so
+    the cell keys, the neighbour-cell cull and the friend tests all meet
+    exact boundary cases; structure and state equal the oracle's."""
+    h = 0.125
+    g = np.arange(-6, 7) * (h / 2)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    x = np.zeros((X.size, 4), np.float32)
+    x[:, 0], x[:, 1], x[:, 2] = X.ravel(), Y.ravel(), Z.ravel()
+    p = pbf_amd.physical_params(len(x), h, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    scene = pbf_amd.Scene(p, pbf_amd.default_options(walls=False), x, np.zeros_like(x))
+    sim, ref = make_pair(scene)
+    for _ in range(2):   # the first substep's build sees the lattice as placed
+        advance(sim, ref, 1, pauses=[1])
+        check_structure(sim, ref)
+        check_state(sim, ref, scene)
+    sim.close()
+
+
 def test_pause_semantics():
     scene = pbf_amd.config("dam4k", seed=9)
     sim, ref = make_pair(scene)
